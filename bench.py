@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Headline benchmark: SpGEMM GFLOPS of C = A^2 for the 3D 27-point stencil
+on a 128^3 periodic grid (n = 2,097,152, 56,623,104 nonzeros, FP64 values
+U[-1,1) from seed 1) on one MI355X.
+
+A step is one full device pipeline (step 1 tile symbolic, step 2 bitmask
+symbolic, step 3 numeric, including the pooled allocations of C) with A
+already resident in HBM; GFLOPS = 2 * nnzCub / t (src/tilespgemm-cuda.h:
+3185).  Prints one JSON line (see the keys below).  --gpus N > 1 exits: this
+build measures one GPU.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rw-spgemm_amd"))
+
+METRIC = "SpGEMM GFLOPS (A², 27-pt 128³); batched/served solves/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def step3_algorithmic_bytes(ta, tc_numtile, tc_nnz, tilen):
+    """Compulsory bytes of one numeric-step launch (the dominant kernel):
+    read A once and B once in tile format (B aliases A for A^2 but is
+    counted again), read the CSC tile index, read C's step-2 outputs
+    (row/col tile index, tile offsets, u8 row offsets, masks), write C's
+    values and in-tile columns."""
+    T, nnz, tilem = ta.numtile, ta.nnz, ta.tilem
+    a_bytes = 4 * (tilem + 1) + 4 * T + 4 * (T + 1) + 8 * nnz + 16 * T + 32 * T
+    b_bytes = a_bytes + 4 * (tilen + 1) + 4 * T + 4 * T
+    c_meta = 4 * tc_numtile * 2 + 4 * (tc_numtile + 1) + 16 * tc_numtile + 32 * tc_numtile
+    c_write = 9 * tc_nnz
+    return a_bytes + b_bytes + c_meta + c_write
+
+
+def cpu_baseline(a, nnzcub_per_row, target_s=15.0):
+    """The reference CPU path (src/spgemm_serialref_spa_new.h, count + fill
+    passes, timed as src/external/cusparse/main.cu:191-222) restated in
+    oracle/oracle.c, on a contiguous block of rows from the middle of the
+    matrix sized to roughly `target_s` seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    mid = a.m // 2
+    rows = 4096
+    t, _ = oracle.spa_structure_rows_timed(a, a, mid, mid + rows)
+    while t < target_s / 8 and rows < a.m // 2:
+        rows = min(a.m // 2, rows * 4)
+        t, _ = oracle.spa_structure_rows_timed(a, a, mid - rows // 2, mid + rows // 2)
+    r0, r1 = mid - rows // 2, mid + rows // 2
+    if rows == 4096:
+        r0, r1 = mid, mid + rows
+    # sample's intermediate products
+    cub = int(nnzcub_per_row[r0:r1].sum())
+    times = [t]
+    for _ in range(2):
+        tt, _ = oracle.spa_structure_rows_timed(a, a, r0, r1)
+        times.append(tt)
+    tmin = min(times)
+    return {"value": round(2.0 * cub / tmin / 1e9, 4), "unit": "GFLOPS", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"rows [{r0},{r1}) of the same A^2 ({r1 - r0} of {a.m} rows, {cub} products), "
+                      f"count+fill passes, min of {len(times)} runs ({tmin:.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dense-threshold", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(SpGEMM would shard by C tile-rows, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import tilespgemm as ts
+
+    N = args.grid
+    a = ts.gen_stencil27(N, N, N, periodic=True, valmode=ts.VAL_UNIFORM_PM1, seed=1)
+    ctx = ts.Context(0)
+    da = ctx.upload(a)
+    ta = ctx.csr2tile(da)
+    ctx.build_csc(ta)
+    nnzcub = ctx.nnzcub(da, da)
+    tune = ts.default_tune()
+    if args.dense_threshold:
+        tune.dense_acc_min_nnz = args.dense_threshold
+    for _ in range(args.warmup):
+        tc, st = ctx.spgemm(ta, ta, tune)
+        ctx.free(tc)
+    ctx.sync()
+    steps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tc, st = ctx.spgemm(ta, ta, tune)
+        steps.append(st.as_dict())
+        ctx.free(tc)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    ev_total = [s["t_total_ms"] for s in steps]
+    ev_s3 = [s["t_step3_ms"] for s in steps]
+    last = steps[-1]
+    gflops = 2.0 * nnzcub / (ms_per_step * 1e6)
+    s3_bytes = step3_algorithmic_bytes(ta, last["numtileC"], last["nnzC"], ta.tilen)
+    s3_ms = statistics.mean(ev_s3)
+    achieved = s3_bytes / (s3_ms * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "step3_traffic_27p128.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"27-point periodic stencil {N}^3, C = A^2", "n": a.m, "nnz": a.nnz,
+                   "nnzCub": nnzcub, "nnzC": last["nnzC"], "tilesC": last["numtileC"],
+                   "tilesC_empty": last["tiles_empty"], "values": "U[-1,1) seed 1"},
+        "roofline": {"bound": "hbm", "kernel": "k_step3 (numeric)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes": s3_bytes, "avg_ms": round(s3_ms, 4)},
+        "step_ms": {"step1": round(statistics.mean(s["t_step1_ms"] for s in steps), 4),
+                    "step2": round(statistics.mean(s["t_step2_ms"] for s in steps), 4),
+                    "step3": round(s3_ms, 4),
+                    "event_total_min": round(min(ev_total), 4),
+                    "event_total_median": round(statistics.median(ev_total), 4)},
+        "gflops_event_min": round(2.0 * nnzcub / (min(ev_total) * 1e6), 2),
+    }
+    ctx.free(ta)
+    ctx.free(da)
+    ctx.close()
+    if not args.no_cpu:
+        rp = a.rowptr.astype(np.int64)
+        rowlen = np.diff(rp)
+        cs = np.concatenate([[0], np.cumsum(rowlen[a.colidx])])
+        cub_row = cs[rp[1:]] - cs[rp[:-1]]
+        out["cpu_baseline"] = cpu_baseline(a, cub_row)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * tsg.h -- device-resident C ABI of the MI355X TileSpGEMM library.
+ *
+ * The reference keeps every operand on the host and copies it to the GPU
+ * inside tilespgemm() (src/tilespgemm-cuda.h:2646-2712).  This interface is
+ * the device-resident layer underneath the drop-in entry points of
+ * tilespgemm.h: matrices live in HBM between calls, conversions run on the
+ * device, and the SpGEMM pipeline is steps 1-3 of src/tilespgemm-cuda.h:
+ * 2740-3124 re-designed for wave64.  All functions return an int status
+ * (TSG_OK = 0, negative on error, message in tsg_last_error()) and never
+ * abort; all device memory is owned by the context's caching allocator.
+ *
+ * Plain C types only (ints, doubles, pointers), so ctypes / cgo / JNI can
+ * bind it directly.
+ */
+#ifndef TSG_H
+#define TSG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TSG_OK = 0,
+    TSG_ERR_HIP = -1,      /* a HIP runtime call failed */
+    TSG_ERR_ARG = -2,      /* invalid argument / shape mismatch */
+    TSG_ERR_OVERFLOW = -3, /* a count does not fit the int32 fields */
+    TSG_ERR_NOMEM = -4,    /* device allocation failed */
+    TSG_ERR_BUDGET = -5,   /* request exceeds the admission budget */
+    TSG_ERR_STATE = -6     /* object used in the wrong state */
+};
+
+typedef struct tsg_ctx tsg_ctx;
+
+/* Context: one device, one HIP stream, a caching device allocator.
+ * Not thread-safe: one host thread per context. */
+int tsg_ctx_create(int device, tsg_ctx **out);
+void tsg_ctx_destroy(tsg_ctx *ctx);
+int tsg_ctx_sync(tsg_ctx *ctx);
+void *tsg_ctx_stream(tsg_ctx *ctx);           /* the hipStream_t */
+long long tsg_ctx_bytes_in_use(tsg_ctx *ctx); /* live allocations */
+long long tsg_ctx_bytes_cached(tsg_ctx *ctx); /* cached free blocks */
+void tsg_ctx_trim(tsg_ctx *ctx);              /* return cached blocks */
+const char *tsg_last_error(void);             /* per host thread */
+
+/* ------------------------------------------------------------------ */
+/* Host CSR (malloc'd arrays, int32 row pointers as MAT_PTR_TYPE).    */
+typedef struct {
+    int m, n, nnz;
+    int *rowptr;    /* m+1 */
+    int *colidx;    /* nnz, ascending inside each row */
+    double *val;    /* nnz */
+} tsg_hcsr;
+void tsg_hcsr_free(tsg_hcsr *h);
+
+/* Value modes for generators and tsg_hcsr_set_values. */
+enum {
+    TSG_VAL_KEEP = 0,     /* generator default (stencil constants) */
+    TSG_VAL_UNIFORM_PM1 = 1, /* U[-1,1) from (seed, entry index) */
+    TSG_VAL_UNIFORM_01 = 2,  /* U(0,1] from (seed, entry index) */
+    TSG_VAL_MOD10 = 3        /* value[i] = i % 10, src/main.cu:191-192 */
+};
+int tsg_hcsr_set_values(tsg_hcsr *h, int mode, unsigned long long seed);
+
+/* 2D 5-point Laplacian on nx*ny (row = x + nx*y), diag 4, off-diag -1. */
+int tsg_gen_stencil5(int nx, int ny, tsg_hcsr *out);
+/* 3D 27-point stencil on nx*ny*nz (row = x + nx*y + nx*ny*z), periodic or
+ * Dirichlet; values by `valmode` (TSG_VAL_KEEP gives 26 / -1). */
+int tsg_gen_stencil27(int nx, int ny, int nz, int periodic, int valmode,
+                      unsigned long long seed, tsg_hcsr *out);
+/* Erdos-Renyi: each row draws Poisson(mean_row) distinct uniform columns. */
+int tsg_gen_er(int n, double mean_row, int valmode, unsigned long long seed,
+               tsg_hcsr *out);
+/* Banded: row i holds every column in [i-halfbw, i+halfbw] inside [0,n). */
+int tsg_gen_banded(int n, int halfbw, int valmode, unsigned long long seed,
+                   tsg_hcsr *out);
+/* R-MAT: 2^scale vertices, edge_factor*2^scale edges drawn with quadrant
+ * probabilities (a,b,c,1-a-b-c); optional symmetrization; duplicates
+ * merged; square CSR with sorted rows. */
+int tsg_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                 int symmetrize, int valmode, unsigned long long seed,
+                 tsg_hcsr *out);
+
+/* ------------------------------------------------------------------ */
+/* Device CSR.                                                        */
+typedef struct {
+    int m, n, nnz;
+    int *rowptr;    /* device, m+1 */
+    int *colidx;    /* device, nnz */
+    double *val;    /* device, nnz */
+} tsg_dcsr;
+
+int tsg_dcsr_upload(tsg_ctx *ctx, const tsg_hcsr *h, tsg_dcsr *out);
+int tsg_dcsr_download(tsg_ctx *ctx, const tsg_dcsr *d, tsg_hcsr *out);
+void tsg_dcsr_free(tsg_ctx *ctx, tsg_dcsr *d);
+/* Stable transpose on the device (rows ascending inside each column). */
+int tsg_dcsr_transpose(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dcsr *at);
+/* sum over nonzeros A(i,k) of nnz(B row k) (src/main.cu:391-397). */
+int tsg_dcsr_nnzcub(tsg_ctx *ctx, const tsg_dcsr *a, const tsg_dcsr *b,
+                    unsigned long long *out);
+
+/* ------------------------------------------------------------------ */
+/* Device tile matrix.  Data arrays are in row-major tile order; the
+ * optional CSC tile index lists, for each tile column, its tiles ordered
+ * by tile row (csc_tileid = row-major tile id).  csr_col holds plain
+ * in-tile columns (0..15).                                             */
+typedef struct {
+    int m, n, tilem, tilen;
+    int numtile, nnz;
+    int *tile_ptr;          /* tilem+1 */
+    int *tile_colidx;       /* numtile */
+    int *tile_rowidx;       /* numtile */
+    int *tile_nnz;          /* numtile+1, exclusive offsets */
+    unsigned char *csr_ptr; /* 16*numtile */
+    unsigned char *csr_col; /* nnz */
+    double *val;            /* nnz */
+    unsigned short *mask;   /* 16*numtile, bit (15-col) */
+    int *csc_ptr;           /* tilen+1   (NULL until built) */
+    int *csc_rowidx;        /* numtile */
+    int *csc_tileid;        /* numtile */
+} tsg_dtiles;
+
+int tsg_csr2tile(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dtiles *out);
+int tsg_tiles_build_csc(tsg_ctx *ctx, tsg_dtiles *t);
+int tsg_tile2csr(tsg_ctx *ctx, const tsg_dtiles *t, tsg_dcsr *out);
+void tsg_tiles_free(tsg_ctx *ctx, tsg_dtiles *t);
+
+/* Host copy of a device tile matrix (malloc'd; NULL pointers skipped). */
+typedef struct {
+    int m, n, tilem, tilen, numtile, nnz;
+    int *tile_ptr, *tile_colidx, *tile_rowidx, *tile_nnz;
+    unsigned char *csr_ptr, *csr_col;
+    double *val;
+    unsigned short *mask;
+    int *csc_ptr, *csc_rowidx, *csc_tileid;
+} tsg_htiles;
+int tsg_tiles_download(tsg_ctx *ctx, const tsg_dtiles *d, tsg_htiles *out);
+void tsg_htiles_free(tsg_htiles *h);
+
+/* ------------------------------------------------------------------ */
+/* SpGEMM C = A*B on device tiles.  B must carry its CSC tile index
+ * (tsg_tiles_build_csc).  Passing the same object as A and B computes A^2
+ * without a second copy of the operand.                               */
+typedef struct {
+    int dense_acc_min_nnz; /* C tiles with nnz >= this use the dense 16x16
+                              LDS accumulator, smaller ones the compact
+                              sparse one (default 64) */
+    int s1_window_words;   /* step-1 LDS bitmap window, 32-bit words per
+                              wave (default 256) */
+    int reserved[6];
+} tsg_tune;
+void tsg_tune_default(tsg_tune *t);
+
+typedef struct {
+    double t_step1_ms, t_step2_ms, t_step3_ms, t_alloc_ms, t_total_ms;
+    long long numtileC, nnzC;
+    long long tiles_sparse_acc, tiles_dense_acc, tiles_empty;
+} tsg_stats;
+
+/* tune and stats may be NULL.  When stats is given, each step is bracketed
+ * by HIP events on the context stream. */
+int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b,
+               const tsg_tune *tune, tsg_dtiles *c, tsg_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* Raw device memory helpers for host layers (all on the context stream;
+ * tsg_h2d / tsg_d2h return after the copy completed).                  */
+int tsg_dmalloc(tsg_ctx *ctx, void **p, size_t bytes);
+void tsg_dfree(tsg_ctx *ctx, void *p);
+int tsg_h2d(tsg_ctx *ctx, void *d, const void *h, size_t bytes);
+int tsg_d2h(tsg_ctx *ctx, void *h, const void *d, size_t bytes);
+int tsg_d_iota(tsg_ctx *ctx, int *d, long long n);
+/* In-tile column bytes packed as (row<<4)|col (A layout of
+ * src/csr2tile.h:185) into d_out[nnz]. */
+int tsg_tiles_packed_cols(tsg_ctx *ctx, const tsg_dtiles *t, unsigned char *d_out);
+/* Copy of t with tile_nnz/csr_ptr/csr_col/val/mask permuted into CSC tile
+ * order (B layout of src/csr2tile.h:327-477); csc_tileid = identity and the
+ * row-major tile index kept for step 1.  Builds t's CSC index if needed. */
+int tsg_tiles_csc_order(tsg_ctx *ctx, tsg_dtiles *t, tsg_dtiles *out);
+
+/* Library build information (compile flags, target). */
+const char *tsg_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSG_H */

@@ -1,0 +1,159 @@
+// tsg_internal.h -- shared internals of the gfx950 TileSpGEMM library:
+// context, caching device allocator, error plumbing and wave64 helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tsg.h"
+
+#define TSG_TILE 16
+
+// ------------------------------------------------------------------ errors
+void tsg_set_error(const char *fmt, ...);
+
+#define TSG_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            tsg_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,         \
+                          hipGetErrorString(e_));                              \
+            return TSG_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+#define TSG_TRY(expr)                                                          \
+    do {                                                                       \
+        int s_ = (expr);                                                       \
+        if (s_ != TSG_OK) return s_;                                           \
+    } while (0)
+
+#define TSG_LAUNCH_CHECK()                                                     \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess) {                                                \
+            tsg_set_error("%s:%d kernel launch -> %s", __FILE__, __LINE__,     \
+                          hipGetErrorString(e_));                              \
+            return TSG_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+// ------------------------------------------------------------------ context
+struct tsg_ctx {
+    int device;
+    hipStream_t stream;
+    // caching allocator: size -> blocks (stream-ordered reuse on `stream`)
+    std::multimap<size_t, void *> free_blocks;
+    std::map<void *, size_t> live;
+    long long bytes_live = 0;
+    long long bytes_cached = 0;
+    // small pinned host buffer for scalar read-backs
+    long long *pinned = nullptr;
+    // persistent small device scratch (scan partials, counters)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int num_cu = 0;
+    size_t lds_per_block = 0;
+};
+
+int tsg_alloc(tsg_ctx *ctx, void **p, size_t bytes);
+void tsg_free(tsg_ctx *ctx, void *p);
+template <typename T>
+static inline int tsg_alloc_t(tsg_ctx *ctx, T **p, size_t count) {
+    return tsg_alloc(ctx, (void **)p, count * sizeof(T));
+}
+int tsg_scratch(tsg_ctx *ctx, size_t bytes, void **p);
+int tsg_read_scalar_i64(tsg_ctx *ctx, const void *dptr, int is64, long long *out);
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n non-negative int32 counts into out[0..n] (out[n] =
+// total).  `in` may alias `out`.  total (may be NULL) is read back to the
+// host; TSG_ERR_OVERFLOW when the total exceeds INT32_MAX.
+int tsg_scan_i32(tsg_ctx *ctx, const int *in, int *out, long long n, long long *total);
+// Same for unsigned-char counts of 16-row tiles is done inside the kernels.
+// Sum of n int64 values (device) -> host.
+int tsg_reduce_u64(tsg_ctx *ctx, const unsigned long long *in, long long n,
+                   unsigned long long *out);
+
+// ------------------------------------------------------------------ sort
+// Stable ascending sort of (key,value) pairs, 32-bit keys, using the
+// library's own LSD radix sort.  Keys must be < (1u << key_bits).
+int tsg_sort_pairs_u32(tsg_ctx *ctx, unsigned *keys, int *vals, long long n,
+                       int key_bits);
+int tsg_sort_pairs_u64(tsg_ctx *ctx, unsigned long long *keys, int *vals, long long n,
+                       int key_bits);
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ void tsg_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int tsg_lane() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T tsg_wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T tsg_wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, 64); v = w < v ? w : v; }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T tsg_wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
+    return v;
+}
+// inclusive scan across the 64 lanes
+template <typename T>
+__device__ __forceinline__ T tsg_wave_incl_scan(T v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+// inclusive scan inside groups of 16 lanes
+template <typename T>
+__device__ __forceinline__ T tsg_group16_incl_scan(T v) {
+    const int r = __lane_id() & 15;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        T w = __shfl_up(v, o, 16);
+        if (r >= o) v += w;
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T tsg_group16_sum(T v) {
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    return v;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X; remap so
+// that each XCD (and its L2) works on one contiguous range of block ids.
+__device__ __forceinline__ unsigned tsg_xcd_block(unsigned bid, unsigned nblocks) {
+    const unsigned nx = 8;
+    const unsigned per = nblocks / nx;      // full rounds
+    const unsigned rem = nblocks % nx;
+    const unsigned xcd = bid % nx, k = bid / nx;
+    // XCD x owns per (+1 if x < rem) consecutive logical ids
+    const unsigned base = xcd * per + (xcd < rem ? xcd : rem);
+    return (k < per + (xcd < rem ? 1u : 0u)) ? base + k : bid;
+}
+
+static inline unsigned tsg_div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+int tsg_validate_csr(tsg_ctx *ctx, const tsg_dcsr *a);

@@ -1,0 +1,384 @@
+"""Python binding of the MI355X TileSpGEMM library (librwspgemm.so).
+
+Mirrors the C ABI of include/tsg.h and include/tilespgemm.h with ctypes.
+The shared library is built in-tree (``make -C rw-spgemm_amd``); importing
+this module without it raises, there is no fallback path.
+
+Typical use::
+
+    import tilespgemm as ts
+    ctx = ts.Context(0)
+    a = ts.gen_stencil27(128, 128, 128, periodic=True, valmode=ts.VAL_UNIFORM_PM1, seed=1)
+    da = ctx.upload(a)
+    ta = ctx.csr2tile(da); ctx.build_csc(ta)
+    tc, stats = ctx.spgemm(ta, ta)          # C = A^2, B aliases A
+    c = ctx.download(ctx.tile2csr(tc))
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librwspgemm.so")
+
+TSG_OK = 0
+VAL_KEEP, VAL_UNIFORM_PM1, VAL_UNIFORM_01, VAL_MOD10 = 0, 1, 2, 3
+
+
+class TsgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"tsg status {status}: {msg}")
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (run make -C rw-spgemm_amd)")
+    return C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+
+
+lib = _load()
+
+_p = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_ull = C.c_ulonglong
+_d = C.c_double
+
+
+class HCsr(C.Structure):
+    _fields_ = [("m", _i), ("n", _i), ("nnz", _i), ("rowptr", C.POINTER(_i)),
+                ("colidx", C.POINTER(_i)), ("val", C.POINTER(_d))]
+
+
+class DCsr(C.Structure):
+    _fields_ = [("m", _i), ("n", _i), ("nnz", _i), ("rowptr", _p), ("colidx", _p), ("val", _p)]
+
+
+class DTiles(C.Structure):
+    _fields_ = [("m", _i), ("n", _i), ("tilem", _i), ("tilen", _i), ("numtile", _i), ("nnz", _i),
+                ("tile_ptr", _p), ("tile_colidx", _p), ("tile_rowidx", _p), ("tile_nnz", _p),
+                ("csr_ptr", _p), ("csr_col", _p), ("val", _p), ("mask", _p),
+                ("csc_ptr", _p), ("csc_rowidx", _p), ("csc_tileid", _p)]
+
+
+class HTiles(C.Structure):
+    _fields_ = [("m", _i), ("n", _i), ("tilem", _i), ("tilen", _i), ("numtile", _i), ("nnz", _i),
+                ("tile_ptr", C.POINTER(_i)), ("tile_colidx", C.POINTER(_i)),
+                ("tile_rowidx", C.POINTER(_i)), ("tile_nnz", C.POINTER(_i)),
+                ("csr_ptr", C.POINTER(C.c_ubyte)), ("csr_col", C.POINTER(C.c_ubyte)),
+                ("val", C.POINTER(_d)), ("mask", C.POINTER(C.c_ushort)),
+                ("csc_ptr", C.POINTER(_i)), ("csc_rowidx", C.POINTER(_i)),
+                ("csc_tileid", C.POINTER(_i))]
+
+
+class Tune(C.Structure):
+    _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("reserved", _i * 6)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("t_step1_ms", _d), ("t_step2_ms", _d), ("t_step3_ms", _d), ("t_alloc_ms", _d),
+                ("t_total_ms", _d), ("numtileC", _ll), ("nnzC", _ll), ("tiles_sparse_acc", _ll),
+                ("tiles_dense_acc", _ll), ("tiles_empty", _ll)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SMatrix(C.Structure):
+    """Same layout as the reference SMatrix (src/common.h:113-135)."""
+    _fields_ = [("m", _i), ("n", _i), ("nnz", _i), ("isSymmetric", _i), ("value", C.POINTER(_d)),
+                ("columnindex", C.POINTER(_i)), ("rowpointer", C.POINTER(_i)), ("tilem", _i),
+                ("tilen", _i), ("tile_ptr", C.POINTER(_i)), ("tile_columnidx", C.POINTER(_i)),
+                ("tile_rowidx", C.POINTER(_i)), ("tile_nnz", C.POINTER(_i)), ("numtile", _i),
+                ("tile_csr_Value", C.POINTER(_d)), ("tile_csr_Col", C.POINTER(C.c_ubyte)),
+                ("tile_csr_Ptr", C.POINTER(C.c_ubyte)), ("mask", C.POINTER(C.c_ushort)),
+                ("csc_tile_ptr", C.POINTER(_i)), ("csc_tile_rowidx", C.POINTER(_i))]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("tsg_last_error", C.c_char_p, [])
+_sig("tsg_build_info", C.c_char_p, [])
+_sig("tsg_ctx_create", _i, [_i, C.POINTER(_p)])
+_sig("tsg_ctx_destroy", None, [_p])
+_sig("tsg_ctx_sync", _i, [_p])
+_sig("tsg_ctx_stream", _p, [_p])
+_sig("tsg_ctx_bytes_in_use", _ll, [_p])
+_sig("tsg_ctx_bytes_cached", _ll, [_p])
+_sig("tsg_ctx_trim", None, [_p])
+_sig("tsg_hcsr_free", None, [C.POINTER(HCsr)])
+_sig("tsg_hcsr_set_values", _i, [C.POINTER(HCsr), _i, _ull])
+_sig("tsg_gen_stencil5", _i, [_i, _i, C.POINTER(HCsr)])
+_sig("tsg_gen_stencil27", _i, [_i, _i, _i, _i, _i, _ull, C.POINTER(HCsr)])
+_sig("tsg_gen_er", _i, [_i, _d, _i, _ull, C.POINTER(HCsr)])
+_sig("tsg_gen_banded", _i, [_i, _i, _i, _ull, C.POINTER(HCsr)])
+_sig("tsg_gen_rmat", _i, [_i, _i, _d, _d, _d, _i, _i, _ull, C.POINTER(HCsr)])
+_sig("tsg_read_mtx", _i, [C.c_char_p, C.POINTER(HCsr), C.POINTER(_i)])
+_sig("tsg_dcsr_upload", _i, [_p, C.POINTER(HCsr), C.POINTER(DCsr)])
+_sig("tsg_dcsr_download", _i, [_p, C.POINTER(DCsr), C.POINTER(HCsr)])
+_sig("tsg_dcsr_free", None, [_p, C.POINTER(DCsr)])
+_sig("tsg_dcsr_transpose", _i, [_p, C.POINTER(DCsr), C.POINTER(DCsr)])
+_sig("tsg_dcsr_nnzcub", _i, [_p, C.POINTER(DCsr), C.POINTER(DCsr), C.POINTER(_ull)])
+_sig("tsg_csr2tile", _i, [_p, C.POINTER(DCsr), C.POINTER(DTiles)])
+_sig("tsg_tiles_build_csc", _i, [_p, C.POINTER(DTiles)])
+_sig("tsg_tile2csr", _i, [_p, C.POINTER(DTiles), C.POINTER(DCsr)])
+_sig("tsg_tiles_free", None, [_p, C.POINTER(DTiles)])
+_sig("tsg_tiles_download", _i, [_p, C.POINTER(DTiles), C.POINTER(HTiles)])
+_sig("tsg_htiles_free", None, [C.POINTER(HTiles)])
+_sig("tsg_tune_default", None, [C.POINTER(Tune)])
+_sig("tsg_spgemm", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), C.POINTER(Tune), C.POINTER(DTiles),
+                        C.POINTER(Stats)])
+_sig("tsg_tiles_packed_cols", _i, [_p, C.POINTER(DTiles), _p])
+_sig("tsg_tiles_csc_order", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles)])
+_sig("tsg_dmalloc", _i, [_p, C.POINTER(_p), C.c_size_t])
+_sig("tsg_dfree", None, [_p, _p])
+_sig("tsg_h2d", _i, [_p, _p, _p, C.c_size_t])
+_sig("tsg_d2h", _i, [_p, _p, _p, C.c_size_t])
+# drop-in reference entry points (include/tilespgemm.h)
+_sig("csr2tile_row_major", None, [C.POINTER(SMatrix)])
+_sig("csr2tile_col_major", None, [C.POINTER(SMatrix)])
+_sig("tile2csr", None, [C.POINTER(SMatrix)])
+_sig("matrix_destroy", None, [C.POINTER(SMatrix)])
+_sig("matrix_transposition", None, [_i, _i, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d),
+                                    C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)])
+_sig("tilespgemm", None, [C.POINTER(SMatrix), C.POINTER(SMatrix), C.POINTER(SMatrix), _p, _p, _i, _d, _d,
+                          _ull, C.POINTER(_ull), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.c_char_p,
+                          C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)])
+_sig("tsg_nnzcub", _ull, [C.POINTER(SMatrix), C.POINTER(SMatrix)])
+_sig("tsg_compat_set_device", _i, [_i])
+
+
+def _check(status):
+    if status != TSG_OK:
+        raise TsgError(status, lib.tsg_last_error().decode(errors="replace"))
+
+
+def build_info():
+    return lib.tsg_build_info().decode()
+
+
+# ---------------------------------------------------------------- host CSR
+class Csr:
+    """Host CSR held in numpy arrays (int32 rowptr/colidx, float64 val)."""
+
+    def __init__(self, m, n, rowptr, colidx, val):
+        self.m, self.n = int(m), int(n)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        assert self.rowptr.shape == (self.m + 1,)
+
+    @property
+    def nnz(self):
+        return int(self.colidx.shape[0])
+
+    def as_c(self):
+        h = HCsr(self.m, self.n, self.nnz, self.rowptr.ctypes.data_as(C.POINTER(_i)),
+                 self.colidx.ctypes.data_as(C.POINTER(_i)), self.val.ctypes.data_as(C.POINTER(_d)))
+        return h
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.colidx, self.rowptr), shape=(self.m, self.n))
+
+    @staticmethod
+    def from_scipy(a):
+        a = a.tocsr()
+        a.sort_indices()
+        return Csr(a.shape[0], a.shape[1], a.indptr, a.indices, a.data)
+
+
+def _take_hcsr(h):
+    """Copy a library-allocated HCsr into numpy and free it."""
+    try:
+        m, n, nnz = h.m, h.n, h.nnz
+        rp = np.ctypeslib.as_array(h.rowptr, shape=(m + 1,)).copy()
+        if nnz:
+            ci = np.ctypeslib.as_array(h.colidx, shape=(nnz,)).copy()
+            va = np.ctypeslib.as_array(h.val, shape=(nnz,)).copy()
+        else:
+            ci = np.zeros(0, np.int32)
+            va = np.zeros(0, np.float64)
+        return Csr(m, n, rp, ci, va)
+    finally:
+        lib.tsg_hcsr_free(C.byref(h))
+
+
+def gen_stencil5(nx, ny):
+    h = HCsr()
+    _check(lib.tsg_gen_stencil5(nx, ny, C.byref(h)))
+    return _take_hcsr(h)
+
+
+def gen_stencil27(nx, ny, nz, periodic=True, valmode=VAL_UNIFORM_PM1, seed=1):
+    h = HCsr()
+    _check(lib.tsg_gen_stencil27(nx, ny, nz, int(periodic), valmode, seed, C.byref(h)))
+    return _take_hcsr(h)
+
+
+def gen_er(n, mean_row, valmode=VAL_UNIFORM_PM1, seed=1):
+    h = HCsr()
+    _check(lib.tsg_gen_er(n, float(mean_row), valmode, seed, C.byref(h)))
+    return _take_hcsr(h)
+
+
+def gen_banded(n, halfbw, valmode=VAL_UNIFORM_PM1, seed=1):
+    h = HCsr()
+    _check(lib.tsg_gen_banded(n, halfbw, valmode, seed, C.byref(h)))
+    return _take_hcsr(h)
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, symmetrize=True, valmode=VAL_UNIFORM_01, seed=1):
+    h = HCsr()
+    _check(lib.tsg_gen_rmat(scale, edge_factor, a, b, c, int(symmetrize), valmode, seed, C.byref(h)))
+    return _take_hcsr(h)
+
+
+def set_values(csr, mode, seed=1):
+    h = csr.as_c()
+    _check(lib.tsg_hcsr_set_values(C.byref(h), mode, seed))
+    return csr
+
+
+def read_mtx(path):
+    h = HCsr()
+    sym = _i(0)
+    _check(lib.tsg_read_mtx(path.encode(), C.byref(h), C.byref(sym)))
+    return _take_hcsr(h), bool(sym.value)
+
+
+# ---------------------------------------------------------------- device
+def _arr(ptr, dtype, n):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype)
+    ctype = np.ctypeslib.as_ctypes_type(dtype)
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(n,)).copy()
+
+
+class Context:
+    """One device, one HIP stream, a caching allocator (tsg_ctx)."""
+
+    def __init__(self, device=0):
+        p = _p()
+        _check(lib.tsg_ctx_create(device, C.byref(p)))
+        self.h = p
+
+    def close(self):
+        if self.h:
+            lib.tsg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib.tsg_ctx_sync(self.h))
+
+    def bytes_in_use(self):
+        return lib.tsg_ctx_bytes_in_use(self.h)
+
+    # CSR
+    def upload(self, csr: Csr) -> DCsr:
+        h = csr.as_c()
+        d = DCsr()
+        _check(lib.tsg_dcsr_upload(self.h, C.byref(h), C.byref(d)))
+        return d
+
+    def download(self, d: DCsr) -> Csr:
+        h = HCsr()
+        _check(lib.tsg_dcsr_download(self.h, C.byref(d), C.byref(h)))
+        return _take_hcsr(h)
+
+    def free(self, obj):
+        if isinstance(obj, DCsr):
+            lib.tsg_dcsr_free(self.h, C.byref(obj))
+        elif isinstance(obj, DTiles):
+            lib.tsg_tiles_free(self.h, C.byref(obj))
+
+    def transpose(self, d: DCsr) -> DCsr:
+        t = DCsr()
+        _check(lib.tsg_dcsr_transpose(self.h, C.byref(d), C.byref(t)))
+        return t
+
+    def nnzcub(self, a: DCsr, b: DCsr) -> int:
+        v = _ull(0)
+        _check(lib.tsg_dcsr_nnzcub(self.h, C.byref(a), C.byref(b), C.byref(v)))
+        return int(v.value)
+
+    # tiles
+    def csr2tile(self, d: DCsr) -> DTiles:
+        t = DTiles()
+        _check(lib.tsg_csr2tile(self.h, C.byref(d), C.byref(t)))
+        return t
+
+    def build_csc(self, t: DTiles):
+        _check(lib.tsg_tiles_build_csc(self.h, C.byref(t)))
+        return t
+
+    def tile2csr(self, t: DTiles) -> DCsr:
+        d = DCsr()
+        _check(lib.tsg_tile2csr(self.h, C.byref(t), C.byref(d)))
+        return d
+
+    def csc_order(self, t: DTiles) -> DTiles:
+        o = DTiles()
+        _check(lib.tsg_tiles_csc_order(self.h, C.byref(t), C.byref(o)))
+        return o
+
+    def packed_cols(self, t: DTiles) -> np.ndarray:
+        p = _p()
+        _check(lib.tsg_dmalloc(self.h, C.byref(p), max(1, t.nnz)))
+        try:
+            _check(lib.tsg_tiles_packed_cols(self.h, C.byref(t), p))
+            out = np.zeros(t.nnz, np.uint8)
+            if t.nnz:
+                _check(lib.tsg_d2h(self.h, out.ctypes.data, p, t.nnz))
+            return out
+        finally:
+            lib.tsg_dfree(self.h, p)
+
+    def download_tiles(self, t: DTiles) -> dict:
+        h = HTiles()
+        _check(lib.tsg_tiles_download(self.h, C.byref(t), C.byref(h)))
+        try:
+            nt = h.numtile
+            out = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, numtile=nt, nnz=h.nnz)
+            out["tile_ptr"] = _arr(h.tile_ptr, np.int32, h.tilem + 1)
+            out["tile_colidx"] = _arr(h.tile_colidx, np.int32, nt)
+            out["tile_rowidx"] = _arr(h.tile_rowidx, np.int32, nt)
+            out["tile_nnz"] = _arr(h.tile_nnz, np.int32, nt + 1)
+            out["csr_ptr"] = _arr(h.csr_ptr, np.uint8, nt * 16)
+            out["csr_col"] = _arr(h.csr_col, np.uint8, h.nnz)
+            out["val"] = _arr(h.val, np.float64, h.nnz)
+            out["mask"] = _arr(h.mask, np.uint16, nt * 16)
+            out["csc_ptr"] = _arr(h.csc_ptr, np.int32, h.tilen + 1) if h.csc_ptr else None
+            out["csc_rowidx"] = _arr(h.csc_rowidx, np.int32, nt) if h.csc_rowidx else None
+            out["csc_tileid"] = _arr(h.csc_tileid, np.int32, nt) if h.csc_tileid else None
+            return out
+        finally:
+            lib.tsg_htiles_free(C.byref(h))
+
+    def spgemm(self, a: DTiles, b: DTiles, tune: Tune | None = None, timed=True):
+        c = DTiles()
+        st = Stats()
+        _check(lib.tsg_spgemm(self.h, C.byref(a), C.byref(b), C.byref(tune) if tune else None, C.byref(c),
+                              C.byref(st) if timed else None))
+        return c, st
+
+
+def default_tune() -> Tune:
+    t = Tune()
+    lib.tsg_tune_default(C.byref(t))
+    return t

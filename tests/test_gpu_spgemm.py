@@ -1,0 +1,116 @@
+"""SpGEMM parity on the GPU: C = A^2 and C = A*A^T through the device C ABI,
+every array of C compared with the oracle (tests/tsg_check.py): step-1 tile
+list, step-2 masks / offsets, C's CSR structure bit-exact, values bit-exact
+(stated bar 1e-12 relative to sum |a_ik b_kj|, asserted too)."""
+import numpy as np
+import pytest
+
+import oracle
+import tilespgemm as ts
+from tsg_check import log, spgemm_check
+
+pytestmark = pytest.mark.gpu
+
+
+def t(csr):
+    tp, tc, tv = oracle.transpose(csr)
+    return ts.Csr(csr.n, csr.m, tp, tc, tv)
+
+
+def test_5pt_64_square_closed_forms(ctx):
+    a = ts.gen_stencil5(64, 64)
+    st = spgemm_check(ctx, a, label="5pt64 A^2")
+    assert st.numtileC == 2652 and st.nnzC == 51972 and st.tiles_empty == 256
+
+
+def test_5pt_64_mod10_values(ctx):
+    a = ts.set_values(ts.gen_stencil5(64, 64), ts.VAL_MOD10)
+    spgemm_check(ctx, a, label="5pt64 i%10")
+
+
+@pytest.mark.parametrize("N,periodic", [(16, True), (24, True), (32, True), (20, False), (33, False)])
+def test_27pt(ctx, N, periodic):
+    a = ts.gen_stencil27(N, N, N, periodic=periodic, valmode=ts.VAL_UNIFORM_PM1, seed=1)
+    st = spgemm_check(ctx, a, label=f"27{'p' if periodic else 'd'}{N}")
+    if periodic:
+        assert st.nnzC == 125 * N ** 3
+
+
+@pytest.mark.parametrize("n,mean,seed", [(1000, 4.0, 1), (5000, 16.0, 2), (3001, 32.0, 3)])
+def test_er_square_and_aat(ctx, n, mean, seed):
+    a = ts.gen_er(n, mean, seed=seed)
+    spgemm_check(ctx, a, label=f"er{n} A^2")
+    spgemm_check(ctx, a, t(a), label=f"er{n} AA^T")
+
+
+@pytest.mark.parametrize("n,bw", [(1000, 2), (4097, 40), (2500, 17)])
+def test_banded(ctx, n, bw):
+    a = ts.gen_banded(n, bw, seed=5)
+    spgemm_check(ctx, a, label=f"banded{n}/{bw}")
+
+
+@pytest.mark.parametrize("scale", [8, 11, 13])
+def test_rmat(ctx, scale):
+    a = ts.gen_rmat(scale, 16, seed=scale)
+    spgemm_check(ctx, a, label=f"rmat{scale} A^2")
+    spgemm_check(ctx, a, t(a), label=f"rmat{scale} AA^T")
+
+
+def test_rectangular_and_edges(ctx):
+    rng = np.random.default_rng(9)
+    import scipy.sparse as sp
+    for (m, k, n, d) in [(37, 300, 21, 0.05), (5, 7, 3, 0.9), (300, 17, 290, 0.2), (16, 16, 16, 1.0), (1, 1, 1, 1.0)]:
+        a = sp.random(m, k, density=d, random_state=rng, format="csr")
+        b = sp.random(k, n, density=d, random_state=rng, format="csr")
+        a.data = rng.uniform(-1, 1, a.nnz)
+        b.data = rng.uniform(-1, 1, b.nnz)
+        spgemm_check(ctx, ts.Csr.from_scipy(a), ts.Csr.from_scipy(b), label=f"rect {m}x{k}x{n}")
+
+
+def test_empty_operands(ctx):
+    z = ts.Csr(64, 64, np.zeros(65, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    st = spgemm_check(ctx, z, label="empty")
+    assert st.numtileC == 0 and st.nnzC == 0
+    a = ts.gen_stencil5(8, 8)
+    zz = ts.Csr(64, 10, np.zeros(65, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    spgemm_check(ctx, a, zz, label="A * empty")
+
+
+def test_dense_block_full_tiles(ctx):
+    """A fully dense 64x64 block: every C tile holds 256 nonzeros (the
+    reference's 'ful' bin) and 64 products per entry."""
+    n = 64
+    rng = np.random.default_rng(3)
+    rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+    ci = np.tile(np.arange(n, dtype=np.int32), n)
+    a = ts.Csr(n, n, rp, ci, rng.uniform(-1, 1, n * n))
+    st = spgemm_check(ctx, a, label="dense64")
+    assert st.tiles_dense_acc == 16 and st.nnzC == n * n
+
+
+@pytest.mark.parametrize("th", [1, 33, 225, 257])
+def test_accumulator_threshold_does_not_change_results(ctx, th):
+    """Every C tile through the dense accumulator (th=1), the reference's
+    lrg/dns split points, or none (257): all bit-identical to the oracle."""
+    tune = ts.default_tune()
+    tune.dense_acc_min_nnz = th
+    a = ts.gen_er(4000, 20.0, seed=8)
+    spgemm_check(ctx, a, tune=tune, label=f"er th={th}")
+    b = ts.gen_stencil27(16, 16, 16, True)
+    spgemm_check(ctx, b, tune=tune, label=f"27p16 th={th}")
+
+
+def test_repeat_is_deterministic(ctx):
+    a = ts.gen_rmat(12, 16, seed=5)
+    da = ctx.upload(a)
+    ta = ctx.csr2tile(da)
+    ctx.build_csc(ta)
+    outs = []
+    for _ in range(3):
+        tc, _ = ctx.spgemm(ta, ta)
+        c = ctx.download_tiles(tc)
+        outs.append(c["val"].view(np.int64).copy())
+        ctx.free(tc)
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    ctx.free(ta)
+    ctx.free(da)
