@@ -149,7 +149,10 @@ typedef struct {
                               sparse one (default 64) */
     int s1_window_words;   /* step-1 LDS bitmap window, 32-bit words per
                               wave (default 256) */
-    int reserved[6];
+    int force_tile_parallel; /* 1: steps 2 and 3 use the tile-parallel
+                              kernels for every tile-row instead of the
+                              row-wise LDS kernels (default 0 = by size) */
+    int reserved[5];
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 
@@ -179,6 +182,10 @@ int tsg_tiles_packed_cols(tsg_ctx *ctx, const tsg_dtiles *t, unsigned char *d_ou
  * order (B layout of src/csr2tile.h:327-477); csc_tileid = identity and the
  * row-major tile index kept for step 1.  Builds t's CSC index if needed. */
 int tsg_tiles_csc_order(tsg_ctx *ctx, tsg_dtiles *t, tsg_dtiles *out);
+/* Inverse: `in` holds tile data in CSC tile order plus both indices (the
+ * csr2tile_col_major layout); out gets the data in row-major tile order
+ * with csc_tileid mapping each CSC position to its row-major tile. */
+int tsg_tiles_from_csc_order(tsg_ctx *ctx, const tsg_dtiles *in, tsg_dtiles *out);
 
 /* Library build information (compile flags, target). */
 const char *tsg_build_info(void);

@@ -256,6 +256,10 @@ __global__ void k_gather_i32(const int *__restrict__ src, const int *__restrict_
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i < n) dst[i] = src[idx[i]];
 }
+__global__ void k_scatter_i32(const int *__restrict__ src, const int *__restrict__ idx, long long n, int *__restrict__ dst) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
 __global__ void k_gather_f64(const double *__restrict__ src, const int *__restrict__ idx, long long n, double *__restrict__ dst) {
     long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i < n) dst[i] = src[idx[i]];
@@ -599,6 +603,108 @@ extern "C" int tsg_tiles_csc_order(tsg_ctx *ctx, tsg_dtiles *t, tsg_dtiles *o) {
     if (nt > 0) {
         hipLaunchKernelGGL(k_csc_data, dim3(tsg_div_up(nt, 256)), dim3(256), 0, s, (int)nt, t->csc_tileid, t->tile_nnz,
                            o->tile_nnz, t->csr_col, t->val, o->csr_col, o->val);
+        TSG_LAUNCH_CHECK();
+    }
+    return TSG_OK;
+}
+
+// Inverse of tsg_tiles_csc_order: tile data given in CSC tile order (the B
+// layout csr2tile_col_major hands to the reference's tilespgemm) is moved to
+// row-major tile order, and csc_tileid maps each CSC position to its
+// row-major tile.  One thread per tile column walks its CSC entries and
+// binary-searches each tile in its tile-row's (sorted) column list.
+__global__ void __launch_bounds__(256)
+k_csc_to_rm(int tilen, const int *__restrict__ csc_ptr, const int *__restrict__ csc_row,
+            const int *__restrict__ tile_ptr, const int *__restrict__ tile_col, const int *__restrict__ nnz_csc,
+            int *__restrict__ rm, int *__restrict__ cnt_rm, int *__restrict__ bad) {
+    const int J = blockIdx.x * 256 + threadIdx.x;
+    if (J >= tilen) return;
+    for (int p = csc_ptr[J]; p < csc_ptr[J + 1]; p++) {
+        const int K = csc_row[p];
+        const int s = tile_ptr[K], len = tile_ptr[K + 1] - s;
+        int lo = 0, hi = len - 1, f = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1, v = tile_col[s + mid];
+            if (v == J) { f = mid; break; }
+            if (v < J) lo = mid + 1; else hi = mid - 1;
+        }
+        if (f < 0) { atomicOr(bad, 1); rm[p] = 0; continue; }
+        rm[p] = s + f;
+        cnt_rm[s + f] = nnz_csc[p + 1] - nnz_csc[p];
+    }
+}
+__global__ void __launch_bounds__(256)
+k_scatter_meta(int nt, const int *__restrict__ rm, const unsigned char *__restrict__ ptr16,
+               const unsigned short *__restrict__ mask, unsigned char *__restrict__ optr16,
+               unsigned short *__restrict__ omask) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nt) return;
+    const long long d = rm[p];
+    *reinterpret_cast<uint4 *>(optr16 + d * 16) = *reinterpret_cast<const uint4 *>(ptr16 + (long long)p * 16);
+    const uint4 *ms = reinterpret_cast<const uint4 *>(mask + (long long)p * 16);
+    uint4 *md = reinterpret_cast<uint4 *>(omask + d * 16);
+    md[0] = ms[0];
+    md[1] = ms[1];
+}
+__global__ void __launch_bounds__(256)
+k_scatter_data(int nt, const int *__restrict__ rm, const int *__restrict__ nnz_csc, const int *__restrict__ nnz_rm,
+               const unsigned char *__restrict__ col, const double *__restrict__ val, unsigned char *__restrict__ ocol,
+               double *__restrict__ oval) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nt) return;
+    const int src = nnz_csc[p], n = nnz_csc[p + 1] - src, dst = nnz_rm[rm[p]];
+    for (int j = 0; j < n; j++) {
+        if (ocol) ocol[dst + j] = col[src + j];
+        oval[dst + j] = val[src + j];
+    }
+}
+
+extern "C" int tsg_tiles_from_csc_order(tsg_ctx *ctx, const tsg_dtiles *in, tsg_dtiles *o) {
+    memset(o, 0, sizeof(*o));
+    if (!in->csc_ptr || !in->csc_rowidx || !in->tile_ptr || !in->tile_colidx) {
+        tsg_set_error("tsg_tiles_from_csc_order: needs the row-major and the CSC tile index");
+        return TSG_ERR_ARG;
+    }
+    const long long nt = in->numtile;
+    o->m = in->m; o->n = in->n; o->tilem = in->tilem; o->tilen = in->tilen;
+    o->numtile = in->numtile; o->nnz = in->nnz;
+    hipStream_t s = ctx->stream;
+    TSG_TRY(tsg_alloc_t(ctx, &o->tile_ptr, (size_t)in->tilem + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->tile_colidx, (size_t)nt + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->tile_rowidx, (size_t)nt + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->csc_ptr, (size_t)in->tilen + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->csc_rowidx, (size_t)nt + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->csc_tileid, (size_t)nt + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->tile_nnz, (size_t)nt + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &o->csr_ptr, (size_t)nt * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &o->mask, (size_t)nt * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &o->val, (size_t)in->nnz + 1));
+    if (in->csr_col) TSG_TRY(tsg_alloc_t(ctx, &o->csr_col, (size_t)in->nnz + 1));
+    TSG_HIP(hipMemcpyAsync(o->tile_ptr, in->tile_ptr, ((size_t)in->tilem + 1) * 4, hipMemcpyDeviceToDevice, s));
+    TSG_HIP(hipMemcpyAsync(o->csc_ptr, in->csc_ptr, ((size_t)in->tilen + 1) * 4, hipMemcpyDeviceToDevice, s));
+    void *scr;
+    TSG_TRY(tsg_scratch(ctx, 64, &scr));
+    TSG_HIP(hipMemsetAsync(scr, 0, 4, s));
+    if (nt > 0) {
+        TSG_HIP(hipMemcpyAsync(o->tile_colidx, in->tile_colidx, nt * 4, hipMemcpyDeviceToDevice, s));
+        TSG_HIP(hipMemcpyAsync(o->csc_rowidx, in->csc_rowidx, nt * 4, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_csc_to_rm, dim3(tsg_div_up(in->tilen, 256)), dim3(256), 0, s, in->tilen, in->csc_ptr,
+                           in->csc_rowidx, in->tile_ptr, in->tile_colidx, in->tile_nnz, o->csc_tileid, o->tile_nnz,
+                           (int *)scr);
+        TSG_LAUNCH_CHECK();
+    }
+    long long bad = 0;
+    TSG_TRY(tsg_read_scalar_i64(ctx, scr, 0, &bad));
+    if (bad) { tsg_set_error("CSC tile index does not match the row-major tile index"); return TSG_ERR_ARG; }
+    TSG_TRY(tsg_scan_i32(ctx, o->tile_nnz, o->tile_nnz, nt, nullptr));
+    if (nt > 0) {
+        // tile row of every row-major tile
+        hipLaunchKernelGGL(k_scatter_i32, dim3(tsg_div_up(nt, 256)), dim3(256), 0, s, in->csc_rowidx, o->csc_tileid, nt,
+                           o->tile_rowidx);
+        hipLaunchKernelGGL(k_scatter_meta, dim3(tsg_div_up(nt, 256)), dim3(256), 0, s, (int)nt, o->csc_tileid,
+                           in->csr_ptr, in->mask, o->csr_ptr, o->mask);
+        hipLaunchKernelGGL(k_scatter_data, dim3(tsg_div_up(nt, 256)), dim3(256), 0, s, (int)nt, o->csc_tileid,
+                           in->tile_nnz, o->tile_nnz, in->csr_col, in->val, o->csr_col, o->val);
         TSG_LAUNCH_CHECK();
     }
     return TSG_OK;

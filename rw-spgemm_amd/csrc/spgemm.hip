@@ -120,19 +120,22 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
 }
 
 // ------------------------------------------------------------------ pairs
-// Operands of steps 2 and 3.
+// Operands of steps 2 and 3.  B's tile data is in row-major tile order;
+// its CSC index (Bcscptr/Bcscrow/Bcscid) lists each tile column's tiles by
+// ascending tile row.
 struct TileOperands {
     const int *Aptr, *Acol, *Annz;
     const unsigned char *Aptr16;
     const unsigned short *Amask;
     const double *Aval;
+    const int *Bptr, *Bcol;
     const int *Bcscptr, *Bcscrow, *Bcscid, *Bnnz;
     const unsigned char *Bptr16;
     const unsigned short *Bmask;
     const double *Bval;
 };
 
-// Largest index p in [0, len) with list[p] == key, or -1.
+// Index p in [0, len) with list[p] == key, or -1 (list ascending).
 __device__ __forceinline__ int find_sorted(const int *__restrict__ list, int len, int key) {
     int lo = 0, hi = len - 1;
     while (lo <= hi) {
@@ -144,19 +147,32 @@ __device__ __forceinline__ int find_sorted(const int *__restrict__ list, int len
     return -1;
 }
 
+#define PAIR_LDS_MAX 64  // both tile lists staged in LDS when each is this short
+
 // Calls body(ta, tb) for every matched pair (A tile ta, B tile tb) of C tile
 // (I, J), in ascending inner tile index K, uniformly across the 16 lanes of
-// the calling group.  `g` is the group (0..3) inside the wave, `r` the lane
-// inside the group.  Must be entered by all 16 lanes of the group.
+// the calling group (g = group in the wave, r = lane in the group).  The
+// keys of the shorter list are binary-searched in the longer one, 16 per
+// round; a 64-bit ballot keeps the matches in key order.  Short lists are
+// first copied to the group's LDS slots (sa/sb) so the searches hit LDS.
 template <typename Body>
-__device__ __forceinline__ void for_each_pair(const TileOperands &op, int I, int J, int g, int r, Body body) {
+__device__ __forceinline__ void for_each_pair(const TileOperands &op, int I, int J, int g, int r, int *sa, int *sb,
+                                              Body body) {
     const int a0 = op.Aptr[I], la = op.Aptr[I + 1] - a0;
     const int b0 = op.Bcscptr[J], lb = op.Bcscptr[J + 1] - b0;
+    if (la == 0 || lb == 0) return;
     const bool a_short = la <= lb;
+    const int ls = a_short ? la : lb, ll = a_short ? lb : la;
     const int *S = a_short ? op.Acol + a0 : op.Bcscrow + b0;
     const int *L = a_short ? op.Bcscrow + b0 : op.Acol + a0;
-    const int ls = a_short ? la : lb, ll = a_short ? lb : la;
-    if (ls == 0) return;
+    const bool staged = la <= PAIR_LDS_MAX && lb <= PAIR_LDS_MAX;
+    if (staged) {
+        for (int x = r; x < ls; x += 16) sa[x] = S[x];
+        for (int x = r; x < ll; x += 16) sb[x] = L[x];
+        tsg_wave_sync();
+        S = sa;
+        L = sb;
+    }
     const int src_base = g * 16;
     for (int base = 0; base < ls; base += 16) {
         const int idx = base + r;
@@ -174,23 +190,145 @@ __device__ __forceinline__ void for_each_pair(const TileOperands &op, int I, int
             body(a0 + apos, op.Bcscid[b0 + bpos]);
         }
     }
+    if (staged) tsg_wave_sync();
 }
 
 // ------------------------------------------------------------------ step 2
+// Row-wise (Gustavson) bitmask symbolic: one 256-thread block per C
+// tile-row.  Every (A tile (I,K), B tile (K,J)) pair of the row -- the same
+// pairs step 1 enumerated -- is visited exactly once, flattened over the
+// threads; no intersection search is needed.  Row r of the pair's product
+// pattern, OR over k in Amask[r] of Bmask[k], is OR-ed into C tile J's row
+// mask held in LDS (OR is order-free, so LDS atomics are exact).
+#define G2_THREADS 256
+#define G2_UMAX 512    // C tiles of one tile-row held in LDS
+#define G2_PCHUNK 256  // A tiles per enumeration round
+
+__global__ void __launch_bounds__(G2_THREADS)
+k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
+             unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz) {
+    __shared__ int s_ccol[G2_UMAX];
+    __shared__ unsigned s_m[G2_UMAX * 8];  // two 16-bit rows per word, even row high
+    __shared__ int s_pre[G2_PCHUNK];
+    __shared__ int s_b0[G2_PCHUNK];
+    __shared__ int s_tot;
+    __shared__ unsigned short s_bst[G2_THREADS][16];
+    const int tid = threadIdx.x;
+    const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
+    if (I >= tilem) return;
+    const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
+    if (u == 0 || u > G2_UMAX) return;  // block-uniform
+    for (int i = tid; i < u; i += G2_THREADS) s_ccol[i] = Ccol[c0 + i];
+    for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
+    const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
+    unsigned short *bst = s_bst[tid];
+    for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
+        const int np = min(G2_PCHUNK, a1 - pc);
+        int len = 0, b0 = 0;
+        if (tid < np) {
+            const int K = op.Acol[pc + tid];
+            b0 = op.Bptr[K];
+            len = op.Bptr[K + 1] - b0;
+        }
+        // block-wide exclusive scan of len
+        __shared__ int s_wsum[G2_THREADS / 64];
+        const int incl = tsg_wave_incl_scan(len);
+        if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < (tid >> 6); w++) woff += s_wsum[w];
+        s_pre[tid] = woff + incl - len;
+        s_b0[tid] = b0;
+        if (tid == G2_THREADS - 1) s_tot = woff + incl;
+        __syncthreads();
+        const int total = s_tot;
+        for (int x = tid; x < total; x += G2_THREADS) {
+            int p = 0;
+#pragma unroll
+            for (int st = G2_PCHUNK / 2; st > 0; st >>= 1)
+                if (p + st < G2_PCHUNK && s_pre[p + st] <= x) p += st;
+            const int q = s_b0[p] + (x - s_pre[p]);
+            const int J = op.Bcol[q];
+            int lo = 0, hi = u - 1;  // J is in the step-1 list
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+            }
+            const int slot = lo;
+            const uint4 *bm4 = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q * 16);
+            const uint4 bA = bm4[0], bB = bm4[1];
+            *reinterpret_cast<uint4 *>(bst) = bA;
+            *reinterpret_cast<uint4 *>(bst + 8) = bB;
+            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)(pc + p) * 16);
+            const uint4 aA = am4[0], aB = am4[1];
+            const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                unsigned am = (aw[r >> 1] >> ((r & 1) * 16)) & 0xffffu;  // little-endian: row 2w low half
+                if (!am) continue;
+                unsigned m = 0;
+                while (am) {
+                    const int b = 31 - __clz(am);
+                    am &= ~(1u << b);
+                    m |= bst[15 - b];
+                }
+                atomicOr(&s_m[slot * 8 + (r >> 1)], m << ((r & 1) ? 0 : 16));
+            }
+        }
+        __syncthreads();
+    }
+    // per C tile: popcounts, u8 row offsets, tile nnz
+    for (int s = tid; s < u; s += G2_THREADS) {
+        unsigned w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = s_m[s * 8 + k];
+        unsigned mk[8], pt[4];
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const unsigned hi = w[k] >> 16, lo = w[k] & 0xffffu;  // rows 2k, 2k+1
+            mk[k] = hi | (lo << 16);                             // little-endian u16 pair
+        }
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+            unsigned o = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int r = q4 * 4 + b;
+                const unsigned row = (r & 1) ? (w[r >> 1] & 0xffffu) : (w[r >> 1] >> 16);
+                o |= (unsigned)sum << (8 * b);
+                sum += __popc(row);
+            }
+            pt[q4] = o;
+        }
+        const long long t = c0 + s;
+        uint4 *cm4 = reinterpret_cast<uint4 *>(Cmask + t * 16);
+        cm4[0] = make_uint4(mk[0], mk[1], mk[2], mk[3]);
+        cm4[1] = make_uint4(mk[4], mk[5], mk[6], mk[7]);
+        *reinterpret_cast<uint4 *>(Cptr16 + t * 16) = make_uint4(pt[0], pt[1], pt[2], pt[3]);
+        Cnnz[t] = sum;
+    }
+}
+
+// Tile-parallel step 2 for tile-rows with more than G2_UMAX C tiles: one C
+// tile per 16-lane group, pairs from the sorted-list intersection.
 __global__ void __launch_bounds__(64 * S23_WAVES)
-k_step2(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ Ccol, int numtileC,
-        unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz,
-        int dense_min, unsigned long long *__restrict__ bins) {
+k_step2_tiles(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Crow,
+              const int *__restrict__ Ccol, int numtileC, unsigned short *__restrict__ Cmask,
+              unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz, int all_rows) {
     __shared__ unsigned short s_bm[S23_WAVES][64];
+    __shared__ int s_la[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
+    __shared__ int s_lb[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const unsigned lb = tsg_xcd_block(blockIdx.x, gridDim.x);
-    const long long t = ((long long)lb * S23_WAVES + wave) * TILES_PER_WAVE + g;
+    const unsigned lblk = tsg_xcd_block(blockIdx.x, gridDim.x);
+    const long long t = ((long long)lblk * S23_WAVES + wave) * TILES_PER_WAVE + g;
     if (t >= numtileC) return;  // whole groups exit together
-    unsigned short *bmg = &s_bm[wave][g * 16];
     const int I = Crow[t], J = Ccol[t];
+    if (!all_rows && Cptr[I + 1] - Cptr[I] <= G2_UMAX) return;  // done by k_step2_rows
+    unsigned short *bmg = &s_bm[wave][g * 16];
     unsigned orow = 0;
-    for_each_pair(op, I, J, g, r, [&](int ta, int tb) {
+    for_each_pair(op, I, J, g, r, s_la[wave][g], s_lb[wave][g], [&](int ta, int tb) {
         bmg[r] = op.Bmask[(long long)tb * 16 + r];
         tsg_wave_sync();
         unsigned am = op.Amask[(long long)ta * 16 + r];
@@ -206,23 +344,72 @@ k_step2(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ C
     const int tot = __shfl(incl, g * 16 + 15, 64);
     Cmask[t * 16 + r] = (unsigned short)orow;
     Cptr16[t * 16 + r] = (unsigned char)(incl - cnt);
-    if (r == 0) {
-        Cnnz[t] = tot;
-        atomicAdd(&bins[tot == 0 ? 0 : (tot >= dense_min ? 2 : 1)], 1ull);
+    if (r == 0) Cnnz[t] = tot;
+}
+
+// Tile counts per accumulator class (statistics only): one atomic per block.
+__global__ void __launch_bounds__(256)
+k_bins(const int *__restrict__ cnt, long long n, int dense_min, unsigned long long *__restrict__ bins) {
+    __shared__ unsigned s[3];
+    if (threadIdx.x < 3) s[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned c0 = 0, c1 = 0, c2 = 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int v = cnt[i];
+        c0 += v == 0;
+        c1 += v > 0 && v < dense_min;
+        c2 += v >= dense_min;
     }
+    c0 = tsg_wave_sum(c0);
+    c1 = tsg_wave_sum(c1);
+    c2 = tsg_wave_sum(c2);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&s[0], c0);
+        atomicAdd(&s[1], c1);
+        atomicAdd(&s[2], c2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&bins[threadIdx.x], (unsigned long long)s[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256)
+k_max_rowlen(const int *__restrict__ ptr, int n, int *__restrict__ out) {
+    int m = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = max(m, ptr[i + 1] - ptr[i]);
+    m = tsg_wave_max(m);
+    __shared__ int s[4];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(s[0], s[1]), max(s[2], s[3])));
 }
 
 // ------------------------------------------------------------------ step 3
+// Tile-rows whose C tiles and nonzeros fit the row-wise kernel's LDS
+// accumulator go through k_step3_rows; the others through the
+// tile-parallel k_step3 below (any size, no LDS limit on the row).
+#define G3_THREADS 256
+#define G3_UMAX 160     // C tiles per tile-row
+#define G3_NNZMAX 2048  // C nonzeros per tile-row (LDS accumulator)
+#define G3_QB 32        // B tiles staged per round
+#define G3_BNNZ 512     // B values staged per round
+
+__device__ __forceinline__ bool g3_row_fits(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int I) {
+    const int c0 = Cptr[I], c1 = Cptr[I + 1];
+    return c1 - c0 <= G3_UMAX && Cnnz[c1] - Cnnz[c0] <= G3_NNZMAX;
+}
+
 #define ACC_STRIDE 17  // doubles per lane row (padding against bank conflicts)
 
 __global__ void __launch_bounds__(64 * S23_WAVES)
-k_step3(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ Ccol, int numtileC,
-        const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
+k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Crow, const int *__restrict__ Ccol,
+        int numtileC, const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
         const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8, double *__restrict__ Cval,
-        int dense_min) {
+        int dense_min, int all_rows) {
     __shared__ double s_acc[S23_WAVES][64 * ACC_STRIDE];
     __shared__ unsigned short s_bm[S23_WAVES][64];
     __shared__ unsigned char s_bp[S23_WAVES][64];
+    __shared__ int s_la[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
+    __shared__ int s_lb[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const unsigned lblk = tsg_xcd_block(blockIdx.x, gridDim.x);
@@ -230,6 +417,7 @@ k_step3(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ C
     if (t >= numtileC) return;
     const int coff = Cnnz[t], cn = Cnnz[t + 1] - coff;
     if (cn == 0) return;  // structurally empty tile (step-1 phantom)
+    if (!all_rows && g3_row_fits(Cptr, Cnnz, Crow[t])) return;  // done by k_step3_rows
     const unsigned cm = Cmask[t * 16 + r];
     const int cp = Cptr16[t * 16 + r];
     const int rowlen = __popc(cm);
@@ -245,7 +433,7 @@ k_step3(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ C
     unsigned short *bmg = &s_bm[wave][g * 16];
     unsigned char *bpg = &s_bp[wave][g * 16];
     const int I = Crow[t], J = Ccol[t];
-    for_each_pair(op, I, J, g, r, [&](int ta, int tb) {
+    for_each_pair(op, I, J, g, r, s_la[wave][g], s_lb[wave][g], [&](int ta, int tb) {
         bmg[r] = op.Bmask[(long long)tb * 16 + r];
         bpg[r] = op.Bptr16[(long long)tb * 16 + r];
         tsg_wave_sync();
@@ -284,6 +472,155 @@ k_step3(TileOperands op, const int *__restrict__ Crow, const int *__restrict__ C
     }
 }
 
+// ------------------------------------------------------------------ step 3, row-wise
+// One 256-thread block per C tile-row whose C tiles and nonzeros fit in
+// LDS.  The tile-row's C values accumulate in LDS in their final order
+// (tile by tile, row by row, ascending column), so the write-out is one
+// contiguous copy.  A tiles (I,K) are taken in ascending K; for each, the
+// A tile and the whole B tile-row K -- contiguous in row-major tile order --
+// are staged into LDS with coalesced loads, then every (B tile q, nonempty
+// A row r) task multiplies A row r by the B tile into C tile slot(q) row r.
+// Tasks of one K touch disjoint (tile, row) accumulators, and a barrier
+// separates consecutive K, so each C entry receives its products in
+// ascending k -- the order of the serial SPA -- with no atomics.
+__global__ void __launch_bounds__(G3_THREADS)
+k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
+             const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
+             const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8, double *__restrict__ Cval) {
+    __shared__ double s_acc[G3_NNZMAX];
+    __shared__ int s_ccol[G3_UMAX];
+    __shared__ int s_coff[G3_UMAX];
+    __shared__ unsigned short s_cmask[G3_UMAX * 16];
+    __shared__ unsigned char s_cptr[G3_UMAX * 16];
+    __shared__ double s_aval[256];
+    __shared__ unsigned short s_am[16];
+    __shared__ unsigned char s_ap[16];
+    __shared__ unsigned char s_rows[16];
+    __shared__ int s_nr;
+    __shared__ double s_bval[G3_BNNZ];
+    __shared__ unsigned short s_bm[G3_QB * 16];
+    __shared__ unsigned char s_bp[G3_QB * 16];
+    __shared__ int s_bnnz[G3_QB + 1];
+    __shared__ int s_slot[G3_QB];
+    __shared__ int s_nq;
+    const int tid = threadIdx.x;
+    const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
+    if (I >= tilem) return;
+    const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
+    if (u == 0 || !g3_row_fits(Cptr, Cnnz, I)) return;  // block-uniform
+    const int nbase = Cnnz[c0], nrow = Cnnz[c0 + u] - nbase;
+    if (nrow == 0) return;
+    for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
+    for (int i = tid; i < u; i += G3_THREADS) {
+        s_ccol[i] = Ccol[c0 + i];
+        s_coff[i] = Cnnz[c0 + i] - nbase;
+    }
+    for (int i = tid; i < u * 16; i += G3_THREADS) {
+        s_cmask[i] = Cmask[(long long)c0 * 16 + i];
+        s_cptr[i] = Cptr16[(long long)c0 * 16 + i];
+    }
+    const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
+    for (int p = a0; p < a1; p++) {
+        const int K = op.Acol[p];
+        const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
+        // stage the A tile
+        const int an0 = op.Annz[p], an = op.Annz[p + 1] - an0;
+        for (int i = tid; i < an; i += G3_THREADS) s_aval[i] = op.Aval[an0 + i];
+        if (tid < 16) {
+            const unsigned short am = op.Amask[(long long)p * 16 + tid];
+            s_am[tid] = am;
+            s_ap[tid] = op.Aptr16[(long long)p * 16 + tid];
+            const unsigned long long bal = __ballot(am != 0);
+            const unsigned rows = (unsigned)bal & 0xffffu;
+            if (am) s_rows[__popc(rows & ((1u << tid) - 1))] = (unsigned char)tid;
+            if (tid == 0) s_nr = __popc(rows);
+        }
+        // B tile-row K in rounds of at most G3_QB tiles / G3_BNNZ values
+        for (int qs = q0; qs < q1;) {
+            __syncthreads();  // previous round's tasks are done with the B stage
+            if (tid < 64) {
+                // tiles qs..qs+i fit while the staged values stay within
+                // G3_BNNZ (offsets are monotone, so the fitting set is a prefix)
+                const bool fit = tid < G3_QB && qs + tid < q1 && op.Bnnz[qs + tid + 1] - op.Bnnz[qs] <= G3_BNNZ;
+                const int n = __popcll(__ballot(fit));
+                if (tid == 0) s_nq = n > 0 ? n : 1;  // one tile (<= 256 values) always fits
+            }
+            __syncthreads();
+            const int nq = s_nq;
+            const int bn0 = op.Bnnz[qs], bn = op.Bnnz[qs + nq] - bn0;
+            for (int i = tid; i < bn; i += G3_THREADS) s_bval[i] = op.Bval[bn0 + i];
+            for (int i = tid; i < nq * 16; i += G3_THREADS) {
+                s_bm[i] = op.Bmask[(long long)qs * 16 + i];
+                s_bp[i] = op.Bptr16[(long long)qs * 16 + i];
+            }
+            for (int i = tid; i <= nq; i += G3_THREADS) s_bnnz[i] = op.Bnnz[qs + i] - bn0;
+            for (int i = tid; i < nq; i += G3_THREADS) {
+                const int J = op.Bcol[qs + i];
+                int lo = 0, hi = u - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+                }
+                s_slot[i] = lo;
+            }
+            __syncthreads();
+            const int nr = s_nr;
+            const int ntask = nq * nr;
+            for (int x = tid; x < ntask; x += G3_THREADS) {
+                const int qi = x / nr, rr = s_rows[x - qi * nr];
+                const int slot = s_slot[qi];
+                const unsigned cm = s_cmask[slot * 16 + rr];
+                const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
+                const unsigned short *bm16 = s_bm + qi * 16;
+                const unsigned char *bp16 = s_bp + qi * 16;
+                const double *bvt = s_bval + s_bnnz[qi];
+                unsigned am = s_am[rr];
+                const double *av = s_aval + s_ap[rr];
+                int ia = 0;
+                while (am) {
+                    const int b = 31 - __clz(am);
+                    am &= ~(1u << b);
+                    const int k = 15 - b;
+                    const double a = av[ia++];
+                    unsigned bm = bm16[k];
+                    const double *brow = bvt + bp16[k];
+                    int jb = 0;
+                    while (bm) {
+                        const int bb = 31 - __clz(bm);
+                        bm &= ~(1u << bb);
+                        double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
+                        *acc = __builtin_fma(a, brow[jb++], *acc);
+                    }
+                }
+            }
+            qs += nq;
+        }
+        __syncthreads();  // K done: next A tile may overwrite the stage
+    }
+    // write out: values are already in final order; columns from the masks
+    for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
+    for (int x = tid; x < u * 16; x += G3_THREADS) {
+        const int slot = x >> 4, rr = x & 15;
+        unsigned cm = s_cmask[x];
+        unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
+        int j = 0;
+        while (cm) {
+            const int bb = 31 - __clz(cm);
+            cm &= ~(1u << bb);
+            oc[j++] = (unsigned char)(15 - bb);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_count_unfit_rows(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem, int *__restrict__ out) {
+    int c = 0;
+    for (int I = blockIdx.x * 256 + threadIdx.x; I < tilem; I += gridDim.x * 256)
+        c += (Cptr[I + 1] > Cptr[I]) && !g3_row_fits(Cptr, Cnnz, I);
+    c = tsg_wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
 // ------------------------------------------------------------------ host
 static int ev_ms(hipEvent_t a, hipEvent_t b, double *ms) {
     float f = 0.f;
@@ -304,7 +641,7 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     if (tune_in) tune = *tune_in; else tsg_tune_default(&tune);
     if (tune.dense_acc_min_nnz < 1) tune.dense_acc_min_nnz = 1;
 
-    hipEvent_t ev[5];
+    hipEvent_t ev[4];
     const bool timed = stats != nullptr;
     if (timed) {
         memset(stats, 0, sizeof(*stats));
@@ -328,10 +665,18 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     C->numtile = (int)numtileC;
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_colidx, (size_t)numtileC + 1));
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_rowidx, (size_t)numtileC + 1));
+    void *scr;
+    TSG_TRY(tsg_alloc(ctx, &scr, 64));
+    unsigned long long *bins = (unsigned long long *)scr;
+    int *maxrow = (int *)(bins + 4);
+    TSG_HIP(hipMemsetAsync(scr, 0, 64, ctx->stream));
     if (numtileC > 0) {
         hipLaunchKernelGGL((k_step1<true, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
                            ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx,
                            C->tile_ptr, C->tile_colidx, C->tile_rowidx);
+        TSG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_max_rowlen, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))), dim3(256), 0,
+                           ctx->stream, C->tile_ptr, C->tilem, maxrow);
         TSG_LAUNCH_CHECK();
     }
     if (timed) TSG_HIP(hipEventRecord(ev[1], ctx->stream));
@@ -339,21 +684,33 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     TSG_TRY(tsg_alloc_t(ctx, &C->mask, (size_t)numtileC * 16 + 16));
     TSG_TRY(tsg_alloc_t(ctx, &C->csr_ptr, (size_t)numtileC * 16 + 16));
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_nnz, (size_t)numtileC + 1));
-    void *scr;
-    TSG_TRY(tsg_alloc(ctx, &scr, 64));
-    unsigned long long *bins = (unsigned long long *)scr;
-    TSG_HIP(hipMemsetAsync(bins, 0, 4 * sizeof(unsigned long long), ctx->stream));
     TileOperands op;
     op.Aptr = A->tile_ptr; op.Acol = A->tile_colidx; op.Annz = A->tile_nnz; op.Aptr16 = A->csr_ptr;
     op.Amask = A->mask; op.Aval = A->val;
+    op.Bptr = B->tile_ptr; op.Bcol = B->tile_colidx;
     op.Bcscptr = B->csc_ptr; op.Bcscrow = B->csc_rowidx; op.Bcscid = B->csc_tileid; op.Bnnz = B->tile_nnz;
     op.Bptr16 = B->csr_ptr; op.Bmask = B->mask; op.Bval = B->val;
     const unsigned tiles_per_block = S23_WAVES * TILES_PER_WAVE;
     const unsigned nblk = tsg_div_up(numtileC, tiles_per_block);
     if (numtileC > 0) {
-        hipLaunchKernelGGL(k_step2, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_rowidx,
-                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, tune.dense_acc_min_nnz, bins);
-        TSG_LAUNCH_CHECK();
+        long long mx = 0;
+        TSG_TRY(tsg_read_scalar_i64(ctx, maxrow, 0, &mx));
+        const int all = tune.force_tile_parallel ? 1 : 0;
+        if (!all) {
+            hipLaunchKernelGGL(k_step2_rows, dim3(C->tilem), dim3(G2_THREADS), 0, ctx->stream, op, C->tile_ptr,
+                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz);
+            TSG_LAUNCH_CHECK();
+        }
+        if (all || mx > G2_UMAX) {
+            hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
+                               C->tile_rowidx, C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, all);
+            TSG_LAUNCH_CHECK();
+        }
+        if (timed) {
+            hipLaunchKernelGGL(k_bins, dim3(std::min<unsigned>(2048, tsg_div_up(numtileC, 256))), dim3(256), 0,
+                               ctx->stream, C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
+            TSG_LAUNCH_CHECK();
+        }
     }
     long long nnzC = 0;
     TSG_TRY(tsg_scan_i32(ctx, C->tile_nnz, C->tile_nnz, numtileC, &nnzC));
@@ -363,10 +720,23 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
     TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
     if (nnzC > 0) {
-        hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_rowidx,
-                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val,
-                           tune.dense_acc_min_nnz);
-        TSG_LAUNCH_CHECK();
+        const int all = tune.force_tile_parallel ? 1 : 0;
+        long long unfit = 1;
+        if (!all) {
+            hipLaunchKernelGGL(k_count_unfit_rows, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))),
+                               dim3(256), 0, ctx->stream, C->tile_ptr, C->tile_nnz, C->tilem, maxrow + 1);
+            TSG_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), 0, ctx->stream, op, C->tile_ptr,
+                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val);
+            TSG_LAUNCH_CHECK();
+            TSG_TRY(tsg_read_scalar_i64(ctx, maxrow + 1, 0, &unfit));
+        }
+        if (unfit > 0) {
+            hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
+                               C->tile_rowidx, C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz,
+                               C->csr_col, C->val, tune.dense_acc_min_nnz, all);
+            TSG_LAUNCH_CHECK();
+        }
     }
     if (timed) {
         TSG_HIP(hipEventRecord(ev[3], ctx->stream));

@@ -212,9 +212,11 @@ void tilespgemm(SMatrix *matrixA, SMatrix *matrixB, SMatrix *matrixC, unsigned i
     (void)densityB;
     (void)filename;
     tsg_ctx *c = tsg_compat_ctx();
-    tsg_dtiles A, B, C;
+    tsg_dtiles A, Bcsc, B, C;
     upload_tiles(c, matrixA, 0, &A);
-    upload_tiles(c, matrixB, 1, &B);
+    upload_tiles(c, matrixB, 1, &Bcsc);
+    CK("B to row-major tile order", tsg_tiles_from_csc_order(c, &Bcsc, &B));
+    tsg_tiles_free(c, &Bcsc);
     double tmin = 1e300;
     tsg_stats st, best;
     memset(&best, 0, sizeof(best));

@@ -76,7 +76,8 @@ class HTiles(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("reserved", _i * 6)]
+    _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("force_tile_parallel", _i),
+                ("reserved", _i * 5)]
 
 
 class Stats(C.Structure):
@@ -139,6 +140,7 @@ _sig("tsg_spgemm", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), C.POINTER(Tune
                         C.POINTER(Stats)])
 _sig("tsg_tiles_packed_cols", _i, [_p, C.POINTER(DTiles), _p])
 _sig("tsg_tiles_csc_order", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles)])
+_sig("tsg_tiles_from_csc_order", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles)])
 _sig("tsg_dmalloc", _i, [_p, C.POINTER(_p), C.c_size_t])
 _sig("tsg_dfree", None, [_p, _p])
 _sig("tsg_h2d", _i, [_p, _p, _p, C.c_size_t])
@@ -335,6 +337,11 @@ class Context:
     def csc_order(self, t: DTiles) -> DTiles:
         o = DTiles()
         _check(lib.tsg_tiles_csc_order(self.h, C.byref(t), C.byref(o)))
+        return o
+
+    def from_csc_order(self, t: DTiles) -> DTiles:
+        o = DTiles()
+        _check(lib.tsg_tiles_from_csc_order(self.h, C.byref(t), C.byref(o)))
         return o
 
     def packed_cols(self, t: DTiles) -> np.ndarray:

@@ -56,6 +56,13 @@ def test_csr2tile_matches_oracle(ctx, name, a):
     sizes = np.diff(ref["tile_nnz"])[order]
     assert np.array_equal(og["tile_nnz"], np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32))
     assert np.array_equal(og["mask"].reshape(-1, 16), ref["mask"].reshape(-1, 16)[order] if len(order) else og["mask"].reshape(-1, 16))
+    # and back to row-major order
+    rm = ctx.from_csc_order(o)
+    rg = ctx.download_tiles(rm)
+    for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "csr_col", "val", "mask"):
+        assert np.array_equal(rg[k], ref[k]), k
+    assert np.array_equal(rg["csc_tileid"], order.astype(np.int32))
+    ctx.free(rm)
     # tile2csr round trip
     back = ctx.download(ctx.tile2csr(t))
     assert np.array_equal(back.rowptr, a.rowptr) and np.array_equal(back.colidx, a.colidx)

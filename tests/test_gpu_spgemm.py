@@ -76,6 +76,29 @@ def test_empty_operands(ctx):
     spgemm_check(ctx, a, zz, label="A * empty")
 
 
+def test_wide_tilerow_uses_tile_parallel_step2(ctx):
+    """A hub row touching 700 tile columns makes C tile-row 0 hold 700 tiles
+    (> the 512 the row-wise step-2 kernel keeps in LDS), so that row goes
+    through the tile-parallel step-2 kernel while the others do not."""
+    n = 16 * 700
+    rows, cols = [], []
+    for i in range(n):
+        cs = {i}
+        if i == 0:
+            cs |= set(range(0, n, 16))
+        if i % 97 == 3:
+            cs |= {(i * 7919) % n, (i * 104729) % n}
+        rows += [i] * len(cs)
+        cols += sorted(cs)
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    rng = np.random.default_rng(1)
+    a = ts.Csr(n, n, rp, np.array(cols, np.int32), rng.uniform(-1, 1, len(cols)))
+    spgemm_check(ctx, a, label="hub row 700 tiles")
+    spgemm_check(ctx, a, t(a), label="hub row 700 tiles AA^T")
+
+
 def test_dense_block_full_tiles(ctx):
     """A fully dense 64x64 block: every C tile holds 256 nonzeros (the
     reference's 'ful' bin) and 64 products per entry."""

@@ -68,6 +68,14 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
         bnd = oracle.spgemm_values(absa, absb, rpc, cc)
         assert np.all(np.abs(c.val - vc) <= 1e-12 * bnd), "values beyond 1e-12"
         assert np.array_equal(c.val.view(np.int64), vc.view(np.int64)), "values not bit-exact"
+    # the tile-parallel kernels must give the identical C (every array)
+    forced = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
+    forced.force_tile_parallel = 1
+    tc2, st2 = ctx.spgemm(ta, tb, forced)
+    C2 = ctx.download_tiles(tc2)
+    for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
+        assert np.array_equal(C2[k], C[k]), f"tile-parallel path differs in {k}"
+    ctx.free(tc2)
     for obj in (dc, tc):
         ctx.free(obj)
     ctx.free(ta)
