@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dense-threshold", type=int, default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first run the plain copy kernel over this many bytes (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -85,6 +87,8 @@ def main():
     N = args.grid
     a = ts.gen_stencil27(N, N, N, periodic=True, valmode=ts.VAL_UNIFORM_PM1, seed=1)
     ctx = ts.Context(0)
+    if args.copy_calibration:
+        ctx.copy_kernel_run(args.copy_calibration)
     da = ctx.upload(a)
     ta = ctx.csr2tile(da)
     ctx.build_csc(ta)

@@ -166,6 +166,8 @@ typedef struct {
  * by HIP events on the context stream. */
 int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b,
                const tsg_tune *tune, tsg_dtiles *c, tsg_stats *stats);
+/* Step 1 alone: number of C tiles (structurally empty ones included). */
+int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, long long *out);
 
 /* ------------------------------------------------------------------ */
 /* Raw device memory helpers for host layers (all on the context stream;
@@ -175,6 +177,9 @@ void tsg_dfree(tsg_ctx *ctx, void *p);
 int tsg_h2d(tsg_ctx *ctx, void *d, const void *h, size_t bytes);
 int tsg_d2h(tsg_ctx *ctx, void *h, const void *d, size_t bytes);
 int tsg_d_iota(tsg_ctx *ctx, int *d, long long n);
+/* Device-to-device copy by a plain streaming kernel (k_copy16); 16-byte
+ * aligned pointers and size.  Used to calibrate byte counters. */
+int tsg_copy_kernel(tsg_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* In-tile column bytes packed as (row<<4)|col (A layout of
  * src/csr2tile.h:185) into d_out[nnz]. */
 int tsg_tiles_packed_cols(tsg_ctx *ctx, const tsg_dtiles *t, unsigned char *d_out);
@@ -186,6 +191,32 @@ int tsg_tiles_csc_order(tsg_ctx *ctx, tsg_dtiles *t, tsg_dtiles *out);
  * csr2tile_col_major layout); out gets the data in row-major tile order
  * with csc_tileid mapping each CSC position to its row-major tile. */
 int tsg_tiles_from_csc_order(tsg_ctx *ctx, const tsg_dtiles *in, tsg_dtiles *out);
+
+/* ------------------------------------------------------------------ */
+/* Orderings and permutations on the device (the reorder path of
+ * src/reorder.h / src/external_reorder.h / reorder_algorithm/).  A
+ * permutation is given as old_id[new] (device int array of n entries);
+ * tsg_perm_invert gives new_id[old].                                  */
+int tsg_perm_validate(tsg_ctx *ctx, int n, const int *d_perm);
+int tsg_perm_invert(tsg_ctx *ctx, int n, const int *d_old_id, int *d_new_id);
+/* ids sorted by (-degree, id); degree = row length, plus the mirrored
+ * count when the matrix stands for a symmetric-header file
+ * (reorder_algorithm/common/order_utils.py:41-45,78-81). */
+int tsg_order_degree(tsg_ctx *ctx, const tsg_dcsr *a, int symmetric_header, int *d_old_id);
+/* Reverse Cuthill-McKee on pattern(A + A^T) (self loops kept; degree =
+ * row length + 1 if the diagonal is stored): seeds and children in
+ * ascending (degree, id).  levels_out (may be NULL) = BFS depth + 1. */
+int tsg_order_rcm(tsg_ctx *ctx, const tsg_dcsr *a, int *d_old_id, int *levels_out);
+/* Seeded uniform random order (keys from a 64-bit mix of (seed, id)). */
+int tsg_order_random(tsg_ctx *ctx, int n, unsigned long long seed, int *d_old_id);
+/* pattern(A + A^T) with sorted rows (values not set). */
+int tsg_dcsr_symmetrize(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dcsr *g);
+/* out = P*A*Q^T: out row r is row d_row_old[r] of A, column c becomes
+ * d_col_new[c], rows sorted by column (src/reorder.h:2429-2504).  With
+ * row map = new_id and column map = old_id it is the restore of
+ * src/reorder.h:2506-2551. */
+int tsg_dcsr_permute(tsg_ctx *ctx, const tsg_dcsr *a, const int *d_row_old, const int *d_col_new,
+                     tsg_dcsr *out);
 
 /* Library build information (compile flags, target). */
 const char *tsg_build_info(void);

@@ -120,6 +120,28 @@ int tsg_scan_i32(tsg_ctx *ctx, const int *in, int *out, long long n, long long *
     return TSG_OK;
 }
 
+// Plain streaming copy (16 B per lane per iteration): the reference point
+// for reading hardware byte counters against a known byte count.
+__global__ void __launch_bounds__(256)
+k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long long n16) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n16; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+extern "C" int tsg_copy_kernel(tsg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) {
+        tsg_set_error("tsg_copy_kernel: 16-byte aligned sizes and pointers only");
+        return TSG_ERR_ARG;
+    }
+    const long long n16 = (long long)(bytes / 16);
+    if (n16 > 0) {
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)std::min<long long>((n16 + 255) / 256, 8192)), dim3(256), 0,
+                           ctx->stream, (const uint4 *)src, (uint4 *)dst, n16);
+        TSG_LAUNCH_CHECK();
+    }
+    return TSG_OK;
+}
+
 __global__ void __launch_bounds__(256)
 k_reduce_u64(const unsigned long long *__restrict__ in, long long n, unsigned long long *out) {
     unsigned long long s = 0;

@@ -388,15 +388,11 @@ k_max_rowlen(const int *__restrict__ ptr, int n, int *__restrict__ out) {
 // accumulator go through k_step3_rows; the others through the
 // tile-parallel k_step3 below (any size, no LDS limit on the row).
 #define G3_THREADS 256
-#define G3_UMAX 160     // C tiles per tile-row
+#define G3_UMAX 128     // C tiles per tile-row
 #define G3_NNZMAX 2048  // C nonzeros per tile-row (LDS accumulator)
-#define G3_QB 32        // B tiles staged per round
-#define G3_BNNZ 512     // B values staged per round
+#define G3_QB 32        // B tiles staged per stage
+#define G3_BNNZ 448     // B values staged per stage
 
-__device__ __forceinline__ bool g3_row_fits(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int I) {
-    const int c0 = Cptr[I], c1 = Cptr[I + 1];
-    return c1 - c0 <= G3_UMAX && Cnnz[c1] - Cnnz[c0] <= G3_NNZMAX;
-}
 
 #define ACC_STRIDE 17  // doubles per lane row (padding against bank conflicts)
 
@@ -404,7 +400,7 @@ __global__ void __launch_bounds__(64 * S23_WAVES)
 k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Crow, const int *__restrict__ Ccol,
         int numtileC, const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
         const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8, double *__restrict__ Cval,
-        int dense_min, int all_rows) {
+        int dense_min, int all_rows, const unsigned char *__restrict__ rowcls) {
     __shared__ double s_acc[S23_WAVES][64 * ACC_STRIDE];
     __shared__ unsigned short s_bm[S23_WAVES][64];
     __shared__ unsigned char s_bp[S23_WAVES][64];
@@ -417,7 +413,7 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
     if (t >= numtileC) return;
     const int coff = Cnnz[t], cn = Cnnz[t + 1] - coff;
     if (cn == 0) return;  // structurally empty tile (step-1 phantom)
-    if (!all_rows && g3_row_fits(Cptr, Cnnz, Crow[t])) return;  // done by k_step3_rows
+    if (!all_rows && rowcls[Crow[t]] == 1) return;  // done by k_step3_rows
     const unsigned cm = Cmask[t * 16 + r];
     const int cp = Cptr16[t * 16 + r];
     const int rowlen = __popc(cm);
@@ -473,155 +469,261 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
 }
 
 // ------------------------------------------------------------------ step 3, row-wise
-// One 256-thread block per C tile-row whose C tiles and nonzeros fit in
-// LDS.  The tile-row's C values accumulate in LDS in their final order
-// (tile by tile, row by row, ascending column), so the write-out is one
-// contiguous copy.  A tiles (I,K) are taken in ascending K; for each, the
-// A tile and the whole B tile-row K -- contiguous in row-major tile order --
-// are staged into LDS with coalesced loads, then every (B tile q, nonempty
-// A row r) task multiplies A row r by the B tile into C tile slot(q) row r.
-// Tasks of one K touch disjoint (tile, row) accumulators, and a barrier
-// separates consecutive K, so each C entry receives its products in
-// ascending k -- the order of the serial SPA -- with no atomics.
+// One 256-thread block per C tile-row whose C tiles, C nonzeros, A tiles and
+// B tile-rows fit in LDS (g3 class, decided by k_row_class).  The
+// tile-row's C values accumulate in LDS in their final order (tile by tile,
+// row by row, ascending column), so the write-out is one contiguous copy.
+// A tiles (I,K) are taken in ascending K; for each, the A tile and the
+// whole B tile-row K -- contiguous in row-major tile order -- are staged
+// into LDS with coalesced loads, and every (B tile q, nonempty A row r)
+// task multiplies A row r by the B tile into C tile slot(q), row r.  Tasks
+// of one K touch disjoint (tile, row) accumulators and a barrier separates
+// consecutive K, so every C entry receives its products in ascending k --
+// the order of the serial SPA -- with no atomics.
+//
+// The staging is software-pipelined: the tile-row's (K, B tile-row) table
+// is loaded once, stage j+1 sits in the second LDS buffer while stage j is
+// computed, and the global loads of stage j+2 are in flight meanwhile, so
+// each K costs one barrier and no exposed global latency.
+#define G3_LAMAX 32  // A tiles per tile-row
+
+// One pipeline stage is held in per-thread registers (av, v0, v1, w, iv).
+// Thread roles: every thread one A value and up to two B values; threads
+// [0,64) B mask chunks (uint4 = 8 rows), [64,96) B row offsets (uint4 = one
+// tile), [128,144) A row mask + offset, [160,193) B tile nnz offsets.
+#define G3_LOAD(j)                                                                                         \
+    do {                                                                                                   \
+        const int p_ = s_row[0][j], pa0_ = s_row[1][j], pan_ = s_row[2][j], q0_ = s_row[3][j];            \
+        const int lb_ = s_row[4][j], pb0_ = s_row[5][j], pbn_ = s_row[6][j];                               \
+        if (tid < pan_) av = op.Aval[pa0_ + tid];                                                          \
+        if (tid < pbn_) v0 = op.Bval[pb0_ + tid];                                                          \
+        if (tid + G3_THREADS < pbn_) v1 = op.Bval[pb0_ + tid + G3_THREADS];                                \
+        if (tid < 64) {                                                                                    \
+            if (tid < 2 * lb_) w = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q0_ * 16)[tid];   \
+        } else if (tid < 96) {                                                                             \
+            if (tid - 64 < lb_) w = reinterpret_cast<const uint4 *>(op.Bptr16 + (long long)q0_ * 16)[tid - 64]; \
+        } else if (tid >= 128 && tid < 144) {                                                              \
+            const long long x_ = (long long)p_ * 16 + (tid - 128);                                         \
+            iv = (int)op.Amask[x_] | ((int)op.Aptr16[x_] << 16);                                           \
+        } else if (tid >= 160 && tid - 160 <= lb_) {                                                       \
+            iv = op.Bnnz[q0_ + tid - 160] - pb0_;                                                          \
+        }                                                                                                  \
+    } while (0)
+
 __global__ void __launch_bounds__(G3_THREADS)
 k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
              const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
-             const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8, double *__restrict__ Cval) {
+             const int *__restrict__ Cnnz, const unsigned char *__restrict__ rowcls,
+             unsigned char *__restrict__ Ccol8, double *__restrict__ Cval) {
     __shared__ double s_acc[G3_NNZMAX];
     __shared__ int s_ccol[G3_UMAX];
     __shared__ int s_coff[G3_UMAX];
-    __shared__ unsigned short s_cmask[G3_UMAX * 16];
-    __shared__ unsigned char s_cptr[G3_UMAX * 16];
-    __shared__ double s_aval[256];
-    __shared__ unsigned short s_am[16];
-    __shared__ unsigned char s_ap[16];
-    __shared__ unsigned char s_rows[16];
-    __shared__ int s_nr;
-    __shared__ double s_bval[G3_BNNZ];
-    __shared__ unsigned short s_bm[G3_QB * 16];
-    __shared__ unsigned char s_bp[G3_QB * 16];
-    __shared__ int s_bnnz[G3_QB + 1];
-    __shared__ int s_slot[G3_QB];
-    __shared__ int s_nq;
+    __shared__ __align__(16) unsigned short s_cmask[G3_UMAX * 16];
+    __shared__ __align__(16) unsigned char s_cptr[G3_UMAX * 16];
+    __shared__ int s_row[8][G3_LAMAX];  // p, A nnz start, A nnz, B tile start, B tiles, B nnz start, B nnz, pair base
+    __shared__ unsigned char s_pslot[G3_LAMAX * G3_QB];
+    __shared__ double s_aval[2][256];
+    __shared__ unsigned short s_am[2][16];
+    __shared__ unsigned char s_ap[2][16];
+    __shared__ unsigned char s_rows[2][16];
+    __shared__ int s_nr[2];
+    __shared__ double s_bval[2][G3_BNNZ];
+    __shared__ __align__(16) unsigned short s_bm[2][G3_QB * 16];
+    __shared__ __align__(16) unsigned char s_bp[2][G3_QB * 16];
+    __shared__ int s_bnnz[2][G3_QB + 1];
     const int tid = threadIdx.x;
     const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
-    if (I >= tilem) return;
+    if (I >= tilem || rowcls[I] != 1) return;  // block-uniform
     const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
-    if (u == 0 || !g3_row_fits(Cptr, Cnnz, I)) return;  // block-uniform
     const int nbase = Cnnz[c0], nrow = Cnnz[c0 + u] - nbase;
-    if (nrow == 0) return;
+    const int a0 = op.Aptr[I], la = op.Aptr[I + 1] - a0;
     for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
     for (int i = tid; i < u; i += G3_THREADS) {
         s_ccol[i] = Ccol[c0 + i];
         s_coff[i] = Cnnz[c0 + i] - nbase;
     }
-    for (int i = tid; i < u * 16; i += G3_THREADS) {
-        s_cmask[i] = Cmask[(long long)c0 * 16 + i];
-        s_cptr[i] = Cptr16[(long long)c0 * 16 + i];
+    for (int i = tid; i < u; i += G3_THREADS) {
+        const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
+        reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
+        reinterpret_cast<uint4 *>(s_cmask)[2 * i + 1] = cm[1];
+        reinterpret_cast<uint4 *>(s_cptr)[i] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + i) * 16)[0];
     }
-    const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-    for (int p = a0; p < a1; p++) {
-        const int K = op.Acol[p];
-        const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
-        // stage the A tile
-        const int an0 = op.Annz[p], an = op.Annz[p + 1] - an0;
-        for (int i = tid; i < an; i += G3_THREADS) s_aval[i] = op.Aval[an0 + i];
-        if (tid < 16) {
-            const unsigned short am = op.Amask[(long long)p * 16 + tid];
-            s_am[tid] = am;
-            s_ap[tid] = op.Aptr16[(long long)p * 16 + tid];
-            const unsigned long long bal = __ballot(am != 0);
-            const unsigned rows = (unsigned)bal & 0xffffu;
-            if (am) s_rows[__popc(rows & ((1u << tid) - 1))] = (unsigned char)tid;
-            if (tid == 0) s_nr = __popc(rows);
+    // row table: one entry per A tile of the tile-row (wave 0)
+    if (tid < 64) {
+        int lb = 0;
+        if (tid < la) {
+            const int p = a0 + tid, K = op.Acol[p];
+            const int pa0 = op.Annz[p], q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
+            const int pb0 = op.Bnnz[q0];
+            lb = q1 - q0;
+            s_row[0][tid] = p;
+            s_row[1][tid] = pa0;
+            s_row[2][tid] = op.Annz[p + 1] - pa0;
+            s_row[3][tid] = q0;
+            s_row[4][tid] = lb;
+            s_row[5][tid] = pb0;
+            s_row[6][tid] = op.Bnnz[q1] - pb0;
         }
-        // B tile-row K in rounds of at most G3_QB tiles / G3_BNNZ values
-        for (int qs = q0; qs < q1;) {
-            __syncthreads();  // previous round's tasks are done with the B stage
-            if (tid < 64) {
-                // tiles qs..qs+i fit while the staged values stay within
-                // G3_BNNZ (offsets are monotone, so the fitting set is a prefix)
-                const bool fit = tid < G3_QB && qs + tid < q1 && op.Bnnz[qs + tid + 1] - op.Bnnz[qs] <= G3_BNNZ;
-                const int n = __popcll(__ballot(fit));
-                if (tid == 0) s_nq = n > 0 ? n : 1;  // one tile (<= 256 values) always fits
+        const int incl = tsg_wave_incl_scan(lb);
+        if (tid < la) s_row[7][tid] = incl - lb;
+        if (tid == 63) s_row[7][G3_LAMAX - 1 < la ? G3_LAMAX - 1 : la] = incl;  // total pairs after the last entry
+    }
+    __syncthreads();
+    // C tile slot of every (A tile, B tile) pair of the tile-row
+    {
+        const int npair = s_row[7][la < G3_LAMAX ? la : G3_LAMAX - 1] + (la == G3_LAMAX ? s_row[4][G3_LAMAX - 1] : 0);
+        for (int x = tid; x < npair; x += G3_THREADS) {
+            int j = 0;
+#pragma unroll
+            for (int st = G3_LAMAX / 2; st > 0; st >>= 1)
+                if (j + st < la && s_row[7][j + st] <= x) j += st;
+            const int J = op.Bcol[s_row[3][j] + (x - s_row[7][j])];
+            int lo = 0, hi = u - 1;  // J is in the step-1 list
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
             }
-            __syncthreads();
-            const int nq = s_nq;
-            const int bn0 = op.Bnnz[qs], bn = op.Bnnz[qs + nq] - bn0;
-            for (int i = tid; i < bn; i += G3_THREADS) s_bval[i] = op.Bval[bn0 + i];
-            for (int i = tid; i < nq * 16; i += G3_THREADS) {
-                s_bm[i] = op.Bmask[(long long)qs * 16 + i];
-                s_bp[i] = op.Bptr16[(long long)qs * 16 + i];
-            }
-            for (int i = tid; i <= nq; i += G3_THREADS) s_bnnz[i] = op.Bnnz[qs + i] - bn0;
-            for (int i = tid; i < nq; i += G3_THREADS) {
-                const int J = op.Bcol[qs + i];
-                int lo = 0, hi = u - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
-                }
-                s_slot[i] = lo;
-            }
-            __syncthreads();
-            const int nr = s_nr;
-            const int ntask = nq * nr;
-            for (int x = tid; x < ntask; x += G3_THREADS) {
-                const int qi = x / nr, rr = s_rows[x - qi * nr];
-                const int slot = s_slot[qi];
-                const unsigned cm = s_cmask[slot * 16 + rr];
-                const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
-                const unsigned short *bm16 = s_bm + qi * 16;
-                const unsigned char *bp16 = s_bp + qi * 16;
-                const double *bvt = s_bval + s_bnnz[qi];
-                unsigned am = s_am[rr];
-                const double *av = s_aval + s_ap[rr];
-                int ia = 0;
-                while (am) {
-                    const int b = 31 - __clz(am);
-                    am &= ~(1u << b);
-                    const int k = 15 - b;
-                    const double a = av[ia++];
-                    unsigned bm = bm16[k];
-                    const double *brow = bvt + bp16[k];
-                    int jb = 0;
-                    while (bm) {
-                        const int bb = 31 - __clz(bm);
-                        bm &= ~(1u << bb);
-                        double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
-                        *acc = __builtin_fma(a, brow[jb++], *acc);
-                    }
-                }
-            }
-            qs += nq;
+            s_pslot[x] = (unsigned char)lo;
         }
-        __syncthreads();  // K done: next A tile may overwrite the stage
+    }
+    double av = 0.0, v0 = 0.0, v1 = 0.0;
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    int iv = 0;
+    auto store = [&](int j, int buf) {
+        const int pan = s_row[2][j], lb = s_row[4][j], pbn = s_row[6][j];
+        if (tid < pan) s_aval[buf][tid] = av;
+        if (tid < pbn) s_bval[buf][tid] = v0;
+        if (tid + G3_THREADS < pbn) s_bval[buf][tid + G3_THREADS] = v1;
+        if (tid < 64) {
+            if (tid < 2 * lb) reinterpret_cast<uint4 *>(s_bm[buf])[tid] = w;
+        } else if (tid < 96) {
+            if (tid - 64 < lb) reinterpret_cast<uint4 *>(s_bp[buf])[tid - 64] = w;
+        } else if (tid >= 160 && tid - 160 <= lb) {
+            s_bnnz[buf][tid - 160] = iv;
+        }
+        if (tid >= 128 && tid < 192) {  // wave 2: A masks, offsets, nonempty-row list
+            const int r = tid - 128;
+            const unsigned am = r < 16 ? (unsigned)iv & 0xffffu : 0u;
+            if (r < 16) {
+                s_am[buf][r] = (unsigned short)am;
+                s_ap[buf][r] = (unsigned char)((unsigned)iv >> 16);
+            }
+            const unsigned rows = (unsigned)__ballot(am != 0) & 0xffffu;
+            if (am) s_rows[buf][__popc(rows & ((1u << r) - 1))] = (unsigned char)r;
+            if (r == 0) s_nr[buf] = __popc(rows);
+        }
+    };
+
+    if (la > 0) {
+        G3_LOAD(0);
+        store(0, 0);
+        if (la > 1) G3_LOAD(1);
+    }
+    __syncthreads();
+    for (int j = 0; j < la; j++) {
+        const int buf = j & 1;
+        const int nr = s_nr[buf], nq = s_row[4][j], pbase = s_row[7][j];
+        const int ntask = nq * nr;
+        // x / nr by multiply-high (exact for x < 2^28, nr <= 16)
+        const unsigned magic = nr > 1 ? (unsigned)((0x100000000ULL + nr - 1) / (unsigned)nr) : 0u;
+        for (int x = tid; x < ntask; x += G3_THREADS) {
+            const int qi = nr > 1 ? (int)__umulhi((unsigned)x, magic) : x;
+            const int rr = s_rows[buf][x - qi * nr];
+            const int slot = s_pslot[pbase + qi];
+            const unsigned cm = s_cmask[slot * 16 + rr];
+            const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
+            const unsigned short *bm16 = s_bm[buf] + qi * 16;
+            const unsigned char *bp16 = s_bp[buf] + qi * 16;
+            const double *bvt = s_bval[buf] + s_bnnz[buf][qi];
+            unsigned am = s_am[buf][rr];
+            const double *av = s_aval[buf] + s_ap[buf][rr];
+            int ia = 0;
+            while (am) {
+                const int b = 31 - __clz(am);
+                am &= ~(1u << b);
+                const int k = 15 - b;
+                const double a = av[ia++];
+                unsigned bm = bm16[k];
+                const double *brow = bvt + bp16[k];
+                int jb = 0;
+                while (bm) {
+                    const int bb = 31 - __clz(bm);
+                    bm &= ~(1u << bb);
+                    double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
+                    *acc = __builtin_fma(a, brow[jb++], *acc);
+                }
+            }
+        }
+        if (j + 1 < la) store(j + 1, buf ^ 1);  // buffer last read by stage j-1
+        if (j + 2 < la) G3_LOAD(j + 2);
+        __syncthreads();
     }
     // write out: values are already in final order; columns from the masks
     for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
     for (int x = tid; x < u * 16; x += G3_THREADS) {
-        const int slot = x >> 4, rr = x & 15;
+        const int slot = x >> 4;
         unsigned cm = s_cmask[x];
         unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
-        int j = 0;
+        int jj = 0;
         while (cm) {
             const int bb = 31 - __clz(cm);
             cm &= ~(1u << bb);
-            oc[j++] = (unsigned char)(15 - bb);
+            oc[jj++] = (unsigned char)(15 - bb);
         }
     }
 }
 
+// Step-3 class of every C tile-row: 1 = row-wise kernel (everything fits
+// its LDS stages), 0 = tile-parallel kernel, 2 = nothing to compute.
+// out[0] += number of class-0 rows.
 __global__ void __launch_bounds__(256)
-k_count_unfit_rows(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem, int *__restrict__ out) {
+k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
+            unsigned char *__restrict__ cls, int *__restrict__ out) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
     int c = 0;
-    for (int I = blockIdx.x * 256 + threadIdx.x; I < tilem; I += gridDim.x * 256)
-        c += (Cptr[I + 1] > Cptr[I]) && !g3_row_fits(Cptr, Cnnz, I);
+    if (I < tilem) {
+        const int c0 = Cptr[I], c1 = Cptr[I + 1];
+        const int nrow = Cnnz[c1] - Cnnz[c0];
+        unsigned char k;
+        if (nrow == 0) {
+            k = 2;
+        } else {
+            const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
+            bool ok = c1 - c0 <= G3_UMAX && nrow <= G3_NNZMAX && a1 - a0 <= G3_LAMAX;
+            for (int p = a0; ok && p < a1; p++) {
+                const int K = op.Acol[p];
+                const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
+                ok = q1 - q0 <= G3_QB && op.Bnnz[q1] - op.Bnnz[q0] <= G3_BNNZ;
+            }
+            k = ok ? 1 : 0;
+        }
+        cls[I] = k;
+        c = k == 0;
+    }
     c = tsg_wave_sum(c);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
 
 // ------------------------------------------------------------------ host
+// Step-1 count only: number of C tiles of A*B (structurally empty ones
+// included), from the row-major tile indices.
+extern "C" int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, long long *out) {
+    if (A->n != B->m) { tsg_set_error("count_tiles: inner dimensions differ"); return TSG_ERR_ARG; }
+    int *cnt;
+    TSG_TRY(tsg_alloc_t(ctx, &cnt, (size_t)A->tilem + 1));
+    if (A->tilem > 0) {
+        hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
+                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
+                           (int *)nullptr, (int *)nullptr);
+        TSG_LAUNCH_CHECK();
+    }
+    long long tot = 0;
+    int st = tsg_scan_i32(ctx, cnt, cnt, A->tilem, &tot);
+    tsg_free(ctx, cnt);
+    *out = tot;
+    return st;
+}
+
 static int ev_ms(hipEvent_t a, hipEvent_t b, double *ms) {
     float f = 0.f;
     TSG_HIP(hipEventElapsedTime(&f, a, b));
@@ -722,21 +824,25 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     if (nnzC > 0) {
         const int all = tune.force_tile_parallel ? 1 : 0;
         long long unfit = 1;
+        unsigned char *rowcls = nullptr;
+        TSG_TRY(tsg_alloc_t(ctx, &rowcls, (size_t)C->tilem + 1));
         if (!all) {
-            hipLaunchKernelGGL(k_count_unfit_rows, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))),
-                               dim3(256), 0, ctx->stream, C->tile_ptr, C->tile_nnz, C->tilem, maxrow + 1);
+            hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, ctx->stream, op,
+                               C->tile_ptr, C->tile_nnz, C->tilem, rowcls, maxrow + 1);
             TSG_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), 0, ctx->stream, op, C->tile_ptr,
-                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val);
+                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col,
+                               C->val);
             TSG_LAUNCH_CHECK();
             TSG_TRY(tsg_read_scalar_i64(ctx, maxrow + 1, 0, &unfit));
         }
         if (unfit > 0) {
             hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
                                C->tile_rowidx, C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz,
-                               C->csr_col, C->val, tune.dense_acc_min_nnz, all);
+                               C->csr_col, C->val, tune.dense_acc_min_nnz, all, (const unsigned char *)rowcls);
             TSG_LAUNCH_CHECK();
         }
+        tsg_free(ctx, rowcls);
     }
     if (timed) {
         TSG_HIP(hipEventRecord(ev[3], ctx->stream));

@@ -145,6 +145,15 @@ _sig("tsg_dmalloc", _i, [_p, C.POINTER(_p), C.c_size_t])
 _sig("tsg_dfree", None, [_p, _p])
 _sig("tsg_h2d", _i, [_p, _p, _p, C.c_size_t])
 _sig("tsg_d2h", _i, [_p, _p, _p, C.c_size_t])
+_sig("tsg_copy_kernel", _i, [_p, _p, _p, C.c_size_t])
+_sig("tsg_spgemm_count_tiles", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), C.POINTER(_ll)])
+_sig("tsg_perm_validate", _i, [_p, _i, _p])
+_sig("tsg_perm_invert", _i, [_p, _i, _p, _p])
+_sig("tsg_order_degree", _i, [_p, C.POINTER(DCsr), _i, _p])
+_sig("tsg_order_rcm", _i, [_p, C.POINTER(DCsr), _p, C.POINTER(_i)])
+_sig("tsg_order_random", _i, [_p, _i, _ull, _p])
+_sig("tsg_dcsr_symmetrize", _i, [_p, C.POINTER(DCsr), C.POINTER(DCsr)])
+_sig("tsg_dcsr_permute", _i, [_p, C.POINTER(DCsr), _p, _p, C.POINTER(DCsr)])
 # drop-in reference entry points (include/tilespgemm.h)
 _sig("csr2tile_row_major", None, [C.POINTER(SMatrix)])
 _sig("csr2tile_col_major", None, [C.POINTER(SMatrix)])
@@ -376,6 +385,83 @@ class Context:
             return out
         finally:
             lib.tsg_htiles_free(C.byref(h))
+
+    # raw device int arrays
+    def dev_ints(self, host: np.ndarray | None = None, n: int | None = None):
+        n = len(host) if host is not None else n
+        p = _p()
+        _check(lib.tsg_dmalloc(self.h, C.byref(p), max(1, n) * 4))
+        if host is not None and n:
+            h = np.ascontiguousarray(host, dtype=np.int32)
+            _check(lib.tsg_h2d(self.h, p, h.ctypes.data, n * 4))
+        return p
+
+    def host_ints(self, p, n):
+        out = np.zeros(n, np.int32)
+        if n:
+            _check(lib.tsg_d2h(self.h, out.ctypes.data, p, n * 4))
+        return out
+
+    def dfree(self, p):
+        lib.tsg_dfree(self.h, p)
+
+    def copy_kernel_run(self, nbytes: int):
+        """One launch of the plain copy kernel over nbytes (byte-counter
+        reference)."""
+        s, d = _p(), _p()
+        _check(lib.tsg_dmalloc(self.h, C.byref(s), nbytes))
+        _check(lib.tsg_dmalloc(self.h, C.byref(d), nbytes))
+        try:
+            _check(lib.tsg_copy_kernel(self.h, d, s, nbytes))
+            self.sync()
+        finally:
+            lib.tsg_dfree(self.h, s)
+            lib.tsg_dfree(self.h, d)
+
+    def count_tiles(self, a: DTiles, b: DTiles) -> int:
+        v = _ll(0)
+        _check(lib.tsg_spgemm_count_tiles(self.h, C.byref(a), C.byref(b), C.byref(v)))
+        return int(v.value)
+
+    def order_degree(self, d: DCsr, symmetric_header=False) -> np.ndarray:
+        p = self.dev_ints(n=d.m)
+        try:
+            _check(lib.tsg_order_degree(self.h, C.byref(d), int(symmetric_header), p))
+            return self.host_ints(p, d.m)
+        finally:
+            self.dfree(p)
+
+    def order_rcm(self, d: DCsr):
+        p = self.dev_ints(n=d.m)
+        lv = _i(0)
+        try:
+            _check(lib.tsg_order_rcm(self.h, C.byref(d), p, C.byref(lv)))
+            return self.host_ints(p, d.m), int(lv.value)
+        finally:
+            self.dfree(p)
+
+    def order_random(self, n, seed) -> np.ndarray:
+        p = self.dev_ints(n=n)
+        try:
+            _check(lib.tsg_order_random(self.h, n, seed, p))
+            return self.host_ints(p, n)
+        finally:
+            self.dfree(p)
+
+    def symmetrize(self, d: DCsr) -> DCsr:
+        g = DCsr()
+        _check(lib.tsg_dcsr_symmetrize(self.h, C.byref(d), C.byref(g)))
+        return g
+
+    def permute(self, d: DCsr, row_old: np.ndarray, col_new: np.ndarray) -> DCsr:
+        pr, pc = self.dev_ints(row_old), self.dev_ints(col_new)
+        try:
+            o = DCsr()
+            _check(lib.tsg_dcsr_permute(self.h, C.byref(d), pr, pc, C.byref(o)))
+            return o
+        finally:
+            self.dfree(pr)
+            self.dfree(pc)
 
     def spgemm(self, a: DTiles, b: DTiles, tune: Tune | None = None, timed=True):
         c = DTiles()
