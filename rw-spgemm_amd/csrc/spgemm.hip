@@ -6,20 +6,23 @@
 //   the (K, J) candidates flattened across the 64 lanes and OR-ed into a
 //   windowed LDS bitmap; count pass, device scan, then a fill pass emitting
 //   the sorted tile columns.  Covers every tilen (no SPA/hash split).
-// Step 2 (16x16 bitmask symbolic, src/tilespgemm-cuda.h:371-755): one C tile
-//   per 16-lane quarter-wave, lane r owns tile row r.  The matched K tiles
-//   come from a binary-search intersection of A's tile-row list and B's
-//   tile-column list, 16 keys per round, ordered with a 64-bit ballot.
-//   Row mask r = OR over pairs, over k in Amask[r], of Bmask[k]; popcount,
-//   in-group scan -> u8 row offsets and tile nnz.
-// Step 3 (numeric, src/tilespgemm-cuda.h:1505-2608): same tile/lane mapping.
-//   Lane r walks its A row in ascending k and each selected B row in
-//   ascending column, accumulating with fma into an LDS accumulator that
-//   only it touches: a 16-wide dense row (tiles with many nonzeros) or the
-//   compact row segment of the C tile (sparse tiles).  No atomics: every C
-//   entry receives its products in ascending inner index k, the order of the
-//   serial SPA reference, so results are deterministic and bit-identical to
-//   it.  (The reference uses FP64 atomicAdd, src/tilespgemm-cuda.h:1670.)
+// Step 2 (16x16 bitmask symbolic, src/tilespgemm-cuda.h:371-755):
+//   k_step2_rows, one block per C tile-row, visits every (A tile, B tile)
+//   pair of the row once (no intersection search) and ORs the pair's
+//   boolean product rows into the C tile masks held in LDS; popcounts give
+//   the u8 row offsets and tile nnz.  Tile-rows with more C tiles than LDS
+//   holds use k_step2_tiles (one C tile per 16-lane group, pairs from a
+//   sorted-list intersection, 64-bit ballots to keep them in K order).
+// Step 3 (numeric, src/tilespgemm-cuda.h:1505-2608): k_step3_rows, one
+//   block per C tile-row, accumulates the row's C values in LDS in their
+//   final order, staging A tile K and B tile-row K per K-step (software
+//   pipelined) and running the (B tile, A row) tasks that meet nonempty B
+//   rows, compacted per wave.  Larger tile-rows use k_step3 (one C tile per
+//   16-lane group, lane r owns tile row r, dense or compact LDS row
+//   accumulator).  No atomics anywhere: every C entry receives its
+//   products in ascending inner index k, the order of the serial SPA
+//   reference, so results are deterministic and bit-identical to it.  (The
+//   reference uses FP64 atomicAdd, src/tilespgemm-cuda.h:1670.)
 #include <string.h>
 
 #include "tsg_internal.h"
@@ -388,10 +391,12 @@ k_max_rowlen(const int *__restrict__ ptr, int n, int *__restrict__ out) {
 // accumulator go through k_step3_rows; the others through the
 // tile-parallel k_step3 below (any size, no LDS limit on the row).
 #define G3_THREADS 256
-#define G3_UMAX 128     // C tiles per tile-row
-#define G3_NNZMAX 2048  // C nonzeros per tile-row (LDS accumulator)
-#define G3_QB 32        // B tiles staged per stage
-#define G3_BNNZ 448     // B values staged per stage
+// Caps of the row-wise kernel (the LDS layout is sized at launch from the
+// actual maxima of the rows that fit, at most ~60 KB per block).
+#define G3_UMAX 256     // C tiles per tile-row (u8 slot ids)
+#define G3_NNZMAX 3072  // C nonzeros per tile-row (LDS accumulator)
+#define G3_QB 32        // B tiles per B tile-row (staging thread roles)
+#define G3_BNNZ 512     // B values per B tile-row (two per thread)
 
 
 #define ACC_STRIDE 17  // doubles per lane row (padding against bank conflicts)
@@ -485,53 +490,146 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
 // is loaded once, stage j+1 sits in the second LDS buffer while stage j is
 // computed, and the global loads of stage j+2 are in flight meanwhile, so
 // each K costs one barrier and no exposed global latency.
-#define G3_LAMAX 32  // A tiles per tile-row
+#define G3_LAMAX 64  // A tiles per tile-row (row table built by wave 0)
 
-// One pipeline stage is held in per-thread registers (av, v0, v1, w, iv).
-// Thread roles: every thread one A value and up to two B values; threads
-// [0,64) B mask chunks (uint4 = 8 rows), [64,96) B row offsets (uint4 = one
-// tile), [128,144) A row mask + offset, [160,193) B tile nnz offsets.
-#define G3_LOAD(j)                                                                                         \
-    do {                                                                                                   \
-        const int p_ = s_row[0][j], pa0_ = s_row[1][j], pan_ = s_row[2][j], q0_ = s_row[3][j];            \
-        const int lb_ = s_row[4][j], pb0_ = s_row[5][j], pbn_ = s_row[6][j];                               \
-        if (tid < pan_) av = op.Aval[pa0_ + tid];                                                          \
-        if (tid < pbn_) v0 = op.Bval[pb0_ + tid];                                                          \
-        if (tid + G3_THREADS < pbn_) v1 = op.Bval[pb0_ + tid + G3_THREADS];                                \
-        if (tid < 64) {                                                                                    \
-            if (tid < 2 * lb_) w = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q0_ * 16)[tid];   \
-        } else if (tid < 96) {                                                                             \
-            if (tid - 64 < lb_) w = reinterpret_cast<const uint4 *>(op.Bptr16 + (long long)q0_ * 16)[tid - 64]; \
-        } else if (tid >= 128 && tid < 144) {                                                              \
-            const long long x_ = (long long)p_ * 16 + (tid - 128);                                         \
-            iv = (int)op.Amask[x_] | ((int)op.Aptr16[x_] << 16);                                           \
-        } else if (tid >= 160 && tid - 160 <= lb_) {                                                       \
-            iv = op.Bnnz[q0_ + tid - 160] - pb0_;                                                          \
-        }                                                                                                  \
+// Dynamic LDS layout of k_step3_rows, from the maxima over its rows.
+struct G3Layout {
+    int umax, nnzmax, lamax, qb, bnnz, amax;
+};
+#define G3_NARR 13
+__host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
+    const size_t sz[G3_NARR] = {
+        8u * (size_t)L.nnzmax,            // 0 acc
+        2u * 8u * (size_t)L.bnnz,         // 1 B values (2 stages)
+        2u * 8u * (size_t)L.amax,         // 2 A values
+        4u * (size_t)L.umax,              // 3 C tile columns
+        4u * (size_t)L.umax,              // 4 C tile offsets
+        32u * (size_t)L.umax,             // 5 C masks
+        16u * (size_t)L.umax,             // 6 C row offsets
+        32u * (size_t)L.lamax,            // 7 row table (8 ints per A tile)
+        (size_t)L.lamax * (size_t)L.qb,   // 8 pair slots
+        2u * 32u * (size_t)L.qb,          // 9 B masks
+        2u * 16u * (size_t)L.qb,          // 10 B row offsets
+        2u * 4u * ((size_t)L.qb + 1),     // 11 B tile nnz offsets
+        2u * 2u * (size_t)L.qb};          // 12 B nonempty-row bits
+    size_t o = 0;
+    for (int i = 0; i < G3_NARR; i++) {
+        off[i] = o;
+        o += (sz[i] + 15) & ~(size_t)15;
+    }
+    return o;
+}
+
+// Pipeline stage registers.  Staging is split by wave so every branch is
+// wave-uniform: wave 0 loads the B tile masks (uint4 = 8 rows, 2 per tile),
+// row offsets (uint4 per tile) and nnz offsets; wave 1 the A tile (values,
+// masks, row offsets); waves 2-3 the B values (4 per lane).  A stage is
+// loaded three K-steps before it is computed (register ring of 3).
+#define G3_SLOT_DECL(S)                                                   \
+    uint4 S##wa = make_uint4(0u, 0u, 0u, 0u), S##wb = make_uint4(0u, 0u, 0u, 0u); \
+    double S##d0 = 0.0, S##d1 = 0.0, S##d2 = 0.0, S##d3 = 0.0;            \
+    int S##iv = 0
+
+#define G3_LOAD(j, S)                                                                                         \
+    do {                                                                                                      \
+        const int lb_ = SROW(4, j), q0_ = SROW(3, j);                                                         \
+        if (wave == 0) {                                                                                      \
+            if (lane < 2 * lb_) S##wa = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q0_ * 16)[lane]; \
+            if (lane < lb_) S##wb = reinterpret_cast<const uint4 *>(op.Bptr16 + (long long)q0_ * 16)[lane];   \
+            if (lane <= lb_) S##iv = op.Bnnz[q0_ + lane] - SROW(5, j);                                        \
+        } else if (wave == 1) {                                                                               \
+            const int pan_ = SROW(2, j), pa0_ = SROW(1, j);                                                   \
+            if (lane < pan_) S##d0 = op.Aval[pa0_ + lane];                                                    \
+            if (lane + 64 < pan_) S##d1 = op.Aval[pa0_ + lane + 64];                                          \
+            if (lane + 128 < pan_) S##d2 = op.Aval[pa0_ + lane + 128];                                        \
+            if (lane + 192 < pan_) S##d3 = op.Aval[pa0_ + lane + 192];                                        \
+            if (lane < 16) {                                                                                  \
+                const long long x_ = (long long)SROW(0, j) * 16 + lane;                                       \
+                S##iv = (int)op.Amask[x_] | ((int)op.Aptr16[x_] << 16);                                       \
+            }                                                                                                 \
+        } else {                                                                                              \
+            const int pbn_ = SROW(6, j), pb0_ = SROW(5, j), b_ = (wave - 2) * 64 + lane;                      \
+            if (b_ < pbn_) S##d0 = op.Bval[pb0_ + b_];                                                        \
+            if (b_ + 128 < pbn_) S##d1 = op.Bval[pb0_ + b_ + 128];                                            \
+            if (b_ + 256 < pbn_) S##d2 = op.Bval[pb0_ + b_ + 256];                                            \
+            if (b_ + 384 < pbn_) S##d3 = op.Bval[pb0_ + b_ + 384];                                            \
+        }                                                                                                     \
     } while (0)
+
+#define G3_STORE(j, S, buf)                                                                                   \
+    do {                                                                                                      \
+        const int lb_ = SROW(4, j);                                                                           \
+        if (wave == 0) {                                                                                      \
+            if (lane < 2 * lb_) reinterpret_cast<uint4 *>(s_bm + (buf) * qb * 16)[lane] = S##wa;               \
+            if (lane < lb_) reinterpret_cast<uint4 *>(s_bp + (buf) * qb * 16)[lane] = S##wb;                   \
+            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = S##iv;                                         \
+            /* nonempty-row bits of this half tile, OR-ed with the partner half */                            \
+            const unsigned wd_[4] = {S##wa.x, S##wa.y, S##wa.z, S##wa.w};                                     \
+            unsigned nz_ = 0;                                                                                 \
+            _Pragma("unroll") for (int i_ = 0; i_ < 4; i_++) {                                                \
+                const int k_ = (lane & 1) * 8 + 2 * i_;                                                       \
+                nz_ |= ((wd_[i_] & 0xffffu) ? 1u : 0u) << (15 - k_);                                          \
+                nz_ |= ((wd_[i_] >> 16) ? 1u : 0u) << (14 - k_);                                              \
+            }                                                                                                 \
+            nz_ |= __shfl_xor(nz_, 1, 64);                                                                    \
+            if (!(lane & 1) && lane < 2 * lb_) s_bnz[(buf) * qb + (lane >> 1)] = (unsigned short)nz_;         \
+        } else if (wave == 1) {                                                                               \
+            const int pan_ = SROW(2, j);                                                                      \
+            double *av_ = s_aval + (buf) * amax;                                                              \
+            if (lane < pan_) av_[lane] = S##d0;                                                               \
+            if (lane + 64 < pan_) av_[lane + 64] = S##d1;                                                     \
+            if (lane + 128 < pan_) av_[lane + 128] = S##d2;                                                   \
+            if (lane + 192 < pan_) av_[lane + 192] = S##d3;                                                   \
+            const unsigned am_ = lane < 16 ? (unsigned)S##iv & 0xffffu : 0u;                                  \
+            if (lane < 16) {                                                                                  \
+                s_am[buf][lane] = (unsigned short)am_;                                                        \
+                s_ap[buf][lane] = (unsigned char)((unsigned)S##iv >> 16);                                     \
+            }                                                                                                 \
+            const unsigned rows_ = (unsigned)__ballot(am_ != 0) & 0xffffu;                                    \
+            if (am_) s_rows[buf][__popc(rows_ & ((1u << lane) - 1))] = (unsigned char)lane;                   \
+            if (lane == 0) s_nr[buf] = __popc(rows_);                                                         \
+        } else {                                                                                              \
+            const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
+            double *bv_ = s_bval + (buf) * bnnzs;                                                             \
+            if (b_ < pbn_) bv_[b_] = S##d0;                                                                   \
+            if (b_ + 128 < pbn_) bv_[b_ + 128] = S##d1;                                                       \
+            if (b_ + 256 < pbn_) bv_[b_ + 256] = S##d2;                                                       \
+            if (b_ + 384 < pbn_) bv_[b_ + 384] = S##d3;                                                       \
+        }                                                                                                     \
+    } while (0)
+
+#define SROW(k, j) s_row[(k) * lamax + (j)]
 
 __global__ void __launch_bounds__(G3_THREADS)
 k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
              const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
              const int *__restrict__ Cnnz, const unsigned char *__restrict__ rowcls,
-             unsigned char *__restrict__ Ccol8, double *__restrict__ Cval) {
-    __shared__ double s_acc[G3_NNZMAX];
-    __shared__ int s_ccol[G3_UMAX];
-    __shared__ int s_coff[G3_UMAX];
-    __shared__ __align__(16) unsigned short s_cmask[G3_UMAX * 16];
-    __shared__ __align__(16) unsigned char s_cptr[G3_UMAX * 16];
-    __shared__ int s_row[8][G3_LAMAX];  // p, A nnz start, A nnz, B tile start, B tiles, B nnz start, B nnz, pair base
-    __shared__ unsigned char s_pslot[G3_LAMAX * G3_QB];
-    __shared__ double s_aval[2][256];
+             unsigned char *__restrict__ Ccol8, double *__restrict__ Cval, G3Layout L) {
+    extern __shared__ __align__(16) unsigned char g3_smem[];
+    size_t off[G3_NARR];
+    g3_layout(L, off);
+    const int lamax = L.lamax, qb = L.qb, amax = L.amax, bnnzs = L.bnnz;
+    double *s_acc = reinterpret_cast<double *>(g3_smem + off[0]);
+    double *s_bval = reinterpret_cast<double *>(g3_smem + off[1]);
+    double *s_aval = reinterpret_cast<double *>(g3_smem + off[2]);
+    int *s_ccol = reinterpret_cast<int *>(g3_smem + off[3]);
+    int *s_coff = reinterpret_cast<int *>(g3_smem + off[4]);
+    unsigned short *s_cmask = reinterpret_cast<unsigned short *>(g3_smem + off[5]);
+    unsigned char *s_cptr = g3_smem + off[6];
+    int *s_row = reinterpret_cast<int *>(g3_smem + off[7]);
+    unsigned char *s_pslot = g3_smem + off[8];
+    unsigned short *s_bm = reinterpret_cast<unsigned short *>(g3_smem + off[9]);
+    unsigned char *s_bp = g3_smem + off[10];
+    int *s_bnnz = reinterpret_cast<int *>(g3_smem + off[11]);
+    unsigned short *s_bnz = reinterpret_cast<unsigned short *>(g3_smem + off[12]);
     __shared__ unsigned short s_am[2][16];
     __shared__ unsigned char s_ap[2][16];
     __shared__ unsigned char s_rows[2][16];
     __shared__ int s_nr[2];
-    __shared__ double s_bval[2][G3_BNNZ];
-    __shared__ __align__(16) unsigned short s_bm[2][G3_QB * 16];
-    __shared__ __align__(16) unsigned char s_bp[2][G3_QB * 16];
-    __shared__ int s_bnnz[2][G3_QB + 1];
+    __shared__ int s_npair;
+    __shared__ unsigned short s_queue[G3_THREADS / 64][128];  // per-wave compacted task list
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
     if (I >= tilem || rowcls[I] != 1) return;  // block-uniform
     const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
@@ -541,104 +639,102 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     for (int i = tid; i < u; i += G3_THREADS) {
         s_ccol[i] = Ccol[c0 + i];
         s_coff[i] = Cnnz[c0 + i] - nbase;
-    }
-    for (int i = tid; i < u; i += G3_THREADS) {
         const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
         reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
         reinterpret_cast<uint4 *>(s_cmask)[2 * i + 1] = cm[1];
         reinterpret_cast<uint4 *>(s_cptr)[i] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + i) * 16)[0];
     }
-    // row table: one entry per A tile of the tile-row (wave 0)
-    if (tid < 64) {
+    // row table: one entry per A tile of the tile-row (wave 0, la <= 64)
+    if (wave == 0) {
         int lb = 0;
-        if (tid < la) {
-            const int p = a0 + tid, K = op.Acol[p];
+        if (lane < la) {
+            const int p = a0 + lane, K = op.Acol[p];
             const int pa0 = op.Annz[p], q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
             const int pb0 = op.Bnnz[q0];
             lb = q1 - q0;
-            s_row[0][tid] = p;
-            s_row[1][tid] = pa0;
-            s_row[2][tid] = op.Annz[p + 1] - pa0;
-            s_row[3][tid] = q0;
-            s_row[4][tid] = lb;
-            s_row[5][tid] = pb0;
-            s_row[6][tid] = op.Bnnz[q1] - pb0;
+            SROW(0, lane) = p;
+            SROW(1, lane) = pa0;
+            SROW(2, lane) = op.Annz[p + 1] - pa0;
+            SROW(3, lane) = q0;
+            SROW(4, lane) = lb;
+            SROW(5, lane) = pb0;
+            SROW(6, lane) = op.Bnnz[q1] - pb0;
         }
         const int incl = tsg_wave_incl_scan(lb);
-        if (tid < la) s_row[7][tid] = incl - lb;
-        if (tid == 63) s_row[7][G3_LAMAX - 1 < la ? G3_LAMAX - 1 : la] = incl;  // total pairs after the last entry
+        if (lane < la) SROW(7, lane) = incl - lb;
+        if (lane == 63) s_npair = incl;
     }
     __syncthreads();
     // C tile slot of every (A tile, B tile) pair of the tile-row
-    {
-        const int npair = s_row[7][la < G3_LAMAX ? la : G3_LAMAX - 1] + (la == G3_LAMAX ? s_row[4][G3_LAMAX - 1] : 0);
-        for (int x = tid; x < npair; x += G3_THREADS) {
-            int j = 0;
+    for (int x = tid; x < s_npair; x += G3_THREADS) {
+        int j = 0;
 #pragma unroll
-            for (int st = G3_LAMAX / 2; st > 0; st >>= 1)
-                if (j + st < la && s_row[7][j + st] <= x) j += st;
-            const int J = op.Bcol[s_row[3][j] + (x - s_row[7][j])];
-            int lo = 0, hi = u - 1;  // J is in the step-1 list
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
-            }
-            s_pslot[x] = (unsigned char)lo;
+        for (int st = G3_LAMAX / 2; st > 0; st >>= 1)
+            if (j + st < la && SROW(7, j + st) <= x) j += st;
+        const int J = op.Bcol[SROW(3, j) + (x - SROW(7, j))];
+        int lo = 0, hi = u - 1;  // J is in the step-1 list
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
         }
+        s_pslot[x] = (unsigned char)lo;
     }
-    double av = 0.0, v0 = 0.0, v1 = 0.0;
-    uint4 w = make_uint4(0u, 0u, 0u, 0u);
-    int iv = 0;
-    auto store = [&](int j, int buf) {
-        const int pan = s_row[2][j], lb = s_row[4][j], pbn = s_row[6][j];
-        if (tid < pan) s_aval[buf][tid] = av;
-        if (tid < pbn) s_bval[buf][tid] = v0;
-        if (tid + G3_THREADS < pbn) s_bval[buf][tid + G3_THREADS] = v1;
-        if (tid < 64) {
-            if (tid < 2 * lb) reinterpret_cast<uint4 *>(s_bm[buf])[tid] = w;
-        } else if (tid < 96) {
-            if (tid - 64 < lb) reinterpret_cast<uint4 *>(s_bp[buf])[tid - 64] = w;
-        } else if (tid >= 160 && tid - 160 <= lb) {
-            s_bnnz[buf][tid - 160] = iv;
-        }
-        if (tid >= 128 && tid < 192) {  // wave 2: A masks, offsets, nonempty-row list
-            const int r = tid - 128;
-            const unsigned am = r < 16 ? (unsigned)iv & 0xffffu : 0u;
-            if (r < 16) {
-                s_am[buf][r] = (unsigned short)am;
-                s_ap[buf][r] = (unsigned char)((unsigned)iv >> 16);
-            }
-            const unsigned rows = (unsigned)__ballot(am != 0) & 0xffffu;
-            if (am) s_rows[buf][__popc(rows & ((1u << r) - 1))] = (unsigned char)r;
-            if (r == 0) s_nr[buf] = __popc(rows);
-        }
-    };
-
+    G3_SLOT_DECL(R0_);
+    G3_SLOT_DECL(R1_);
+    G3_SLOT_DECL(R2_);
     if (la > 0) {
-        G3_LOAD(0);
-        store(0, 0);
-        if (la > 1) G3_LOAD(1);
+        G3_LOAD(0, R0_);
+        if (la > 1) G3_LOAD(1, R1_);
+        if (la > 2) G3_LOAD(2, R2_);
+        G3_STORE(0, R0_, 0);
     }
     __syncthreads();
-    for (int j = 0; j < la; j++) {
+
+    // compute K-step j from buffer j&1
+    auto compute = [&](int j) {
         const int buf = j & 1;
-        const int nr = s_nr[buf], nq = s_row[4][j], pbase = s_row[7][j];
-        const int ntask = nq * nr;
+        const int nr = s_nr[buf], nq = SROW(4, j), pbase = SROW(7, j);
+        const int ncand = nq * nr;
         // x / nr by multiply-high (exact for x < 2^28, nr <= 16)
         const unsigned magic = nr > 1 ? (unsigned)((0x100000000ULL + nr - 1) / (unsigned)nr) : 0u;
-        for (int x = tid; x < ntask; x += G3_THREADS) {
-            const int qi = nr > 1 ? (int)__umulhi((unsigned)x, magic) : x;
-            const int rr = s_rows[buf][x - qi * nr];
+        const unsigned short *sbz = s_bnz + buf * qb;
+        // candidate tasks (B tile qi, A row rr) meeting at least one nonempty
+        // B row, compacted per wave (at most two chunks of 64 per wave)
+        int qn = 0;
+        for (int c = wave * 64; c < ncand; c += G3_THREADS) {
+            const int x = c + lane;
+            int code = 0;
+            bool act = false;
+            if (x < ncand) {
+                const int qi = nr > 1 ? (int)__umulhi((unsigned)x, magic) : x;
+                const int rr = s_rows[buf][x - qi * nr];
+                act = (s_am[buf][rr] & sbz[qi]) != 0;
+                code = (qi << 4) | rr;
+            }
+            const unsigned long long bal = __ballot(act);
+            if (act) s_queue[wave][qn + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)code;
+            qn += __popcll(bal);
+        }
+        tsg_wave_sync();
+        const unsigned short *sbm = s_bm + buf * qb * 16;
+        const unsigned char *sbp = s_bp + buf * qb * 16;
+        const int *sbnnz = s_bnnz + buf * (qb + 1);
+        const double *sbv = s_bval + buf * bnnzs;
+        const double *sav = s_aval + buf * amax;
+        for (int t0 = 0; t0 < qn; t0 += 64) {
+            if (t0 + lane >= qn) break;
+            const int code = s_queue[wave][t0 + lane];
+            const int qi = code >> 4, rr = code & 15;
             const int slot = s_pslot[pbase + qi];
             const unsigned cm = s_cmask[slot * 16 + rr];
             const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
-            const unsigned short *bm16 = s_bm[buf] + qi * 16;
-            const unsigned char *bp16 = s_bp[buf] + qi * 16;
-            const double *bvt = s_bval[buf] + s_bnnz[buf][qi];
+            const unsigned short *bm16 = sbm + qi * 16;
+            const unsigned char *bp16 = sbp + qi * 16;
+            const double *bvt = sbv + sbnnz[qi];
             unsigned am = s_am[buf][rr];
-            const double *av = s_aval[buf] + s_ap[buf][rr];
+            const double *av = sav + s_ap[buf][rr];
             int ia = 0;
-            while (am) {
+            while (am) {  // A row entries, ascending k
                 const int b = 31 - __clz(am);
                 am &= ~(1u << b);
                 const int k = 15 - b;
@@ -646,7 +742,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
                 unsigned bm = bm16[k];
                 const double *brow = bvt + bp16[k];
                 int jb = 0;
-                while (bm) {
+                while (bm) {  // B row k entries, ascending column
                     const int bb = 31 - __clz(bm);
                     bm &= ~(1u << bb);
                     double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
@@ -654,10 +750,20 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
                 }
             }
         }
-        if (j + 1 < la) store(j + 1, buf ^ 1);  // buffer last read by stage j-1
-        if (j + 2 < la) G3_LOAD(j + 2);
-        __syncthreads();
+    };
+#define G3_ITER(j, SN, SR)                                           \
+    if ((j) < la) {                                                  \
+        compute(j);                                                  \
+        if ((j) + 1 < la) G3_STORE((j) + 1, SN, ((j) + 1) & 1);      \
+        if ((j) + 3 < la) G3_LOAD((j) + 3, SR);                      \
+        __syncthreads();                                             \
     }
+    for (int j0 = 0; j0 < la; j0 += 3) {
+        G3_ITER(j0, R1_, R0_)
+        G3_ITER(j0 + 1, R2_, R1_)
+        G3_ITER(j0 + 2, R0_, R2_)
+    }
+#undef G3_ITER
     // write out: values are already in final order; columns from the masks
     for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
     for (int x = tid; x < u * 16; x += G3_THREADS) {
@@ -672,15 +778,18 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         }
     }
 }
+#undef SROW
 
-// Step-3 class of every C tile-row: 1 = row-wise kernel (everything fits
-// its LDS stages), 0 = tile-parallel kernel, 2 = nothing to compute.
-// out[0] += number of class-0 rows.
+// Step-3 class of every C tile-row: 1 = row-wise kernel (its C tiles, C
+// nonzeros, A tiles and B tile-rows are within the G3 caps), 0 = tile-parallel
+// kernel, 2 = nothing to compute.  out[0] += class-0 rows; out[1..6] = max
+// over class-1 rows of (C tiles, C nonzeros, A tiles, B tiles per B
+// tile-row, B nonzeros per B tile-row, A tile nonzeros) for the LDS layout.
 __global__ void __launch_bounds__(256)
 k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
             unsigned char *__restrict__ cls, int *__restrict__ out) {
     const int I = blockIdx.x * 256 + threadIdx.x;
-    int c = 0;
+    int unfit = 0, mu = 0, mn = 0, ml = 0, mq = 0, mb = 0, ma = 0;
     if (I < tilem) {
         const int c0 = Cptr[I], c1 = Cptr[I + 1];
         const int nrow = Cnnz[c1] - Cnnz[c0];
@@ -690,18 +799,32 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
         } else {
             const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
             bool ok = c1 - c0 <= G3_UMAX && nrow <= G3_NNZMAX && a1 - a0 <= G3_LAMAX;
+            int lq = 0, lbn = 0, lan = 0;
             for (int p = a0; ok && p < a1; p++) {
                 const int K = op.Acol[p];
                 const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
-                ok = q1 - q0 <= G3_QB && op.Bnnz[q1] - op.Bnnz[q0] <= G3_BNNZ;
+                const int bn = op.Bnnz[q1] - op.Bnnz[q0];
+                ok = q1 - q0 <= G3_QB && bn <= G3_BNNZ;
+                lq = max(lq, q1 - q0);
+                lbn = max(lbn, bn);
+                lan = max(lan, op.Annz[p + 1] - op.Annz[p]);
             }
             k = ok ? 1 : 0;
+            if (ok) {
+                mu = c1 - c0; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
+            }
         }
         cls[I] = k;
-        c = k == 0;
+        unfit = k == 0;
     }
-    c = tsg_wave_sum(c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+    unfit = tsg_wave_sum(unfit);
+    mu = tsg_wave_max(mu); mn = tsg_wave_max(mn); ml = tsg_wave_max(ml);
+    mq = tsg_wave_max(mq); mb = tsg_wave_max(mb); ma = tsg_wave_max(ma);
+    if ((threadIdx.x & 63) == 0) {
+        if (unfit) atomicAdd(&out[0], unfit);
+        atomicMax(&out[1], mu); atomicMax(&out[2], mn); atomicMax(&out[3], ml);
+        atomicMax(&out[4], mq); atomicMax(&out[5], mb); atomicMax(&out[6], ma);
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -825,16 +948,33 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
         const int all = tune.force_tile_parallel ? 1 : 0;
         long long unfit = 1;
         unsigned char *rowcls = nullptr;
+        int *cls_out = nullptr;
         TSG_TRY(tsg_alloc_t(ctx, &rowcls, (size_t)C->tilem + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &cls_out, 8));
         if (!all) {
+            TSG_HIP(hipMemsetAsync(cls_out, 0, 8 * sizeof(int), ctx->stream));
             hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, ctx->stream, op,
-                               C->tile_ptr, C->tile_nnz, C->tilem, rowcls, maxrow + 1);
+                               C->tile_ptr, C->tile_nnz, C->tilem, rowcls, cls_out);
             TSG_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), 0, ctx->stream, op, C->tile_ptr,
-                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col,
-                               C->val);
-            TSG_LAUNCH_CHECK();
-            TSG_TRY(tsg_read_scalar_i64(ctx, maxrow + 1, 0, &unfit));
+            int h[8];
+            TSG_HIP(hipMemcpyAsync(h, cls_out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+            TSG_HIP(hipStreamSynchronize(ctx->stream));
+            unfit = h[0];
+            G3Layout L;
+            L.umax = std::max(1, h[1]);
+            L.nnzmax = std::max(1, h[2]);
+            L.lamax = std::max(1, h[3]);
+            L.qb = std::max(1, h[4]);
+            L.bnnz = std::max(1, h[5]);
+            L.amax = std::max(1, h[6]);
+            size_t off[G3_NARR];
+            const size_t lds = g3_layout(L, off);
+            if (h[1] > 0) {
+                hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), lds, ctx->stream, op, C->tile_ptr,
+                                   C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col,
+                                   C->val, L);
+                TSG_LAUNCH_CHECK();
+            }
         }
         if (unfit > 0) {
             hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
@@ -843,6 +983,7 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
             TSG_LAUNCH_CHECK();
         }
         tsg_free(ctx, rowcls);
+        tsg_free(ctx, cls_out);
     }
     if (timed) {
         TSG_HIP(hipEventRecord(ev[3], ctx->stream));
