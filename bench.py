@@ -66,6 +66,86 @@ def cpu_baseline(a, nnzcub_per_row, target_s=15.0):
                       f"count+fill passes, min of {len(times)} runs ({tmin:.2f} s)"}
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotune=True):
+    """Batched workload: `count` SpGEMMs (A^2 / AA^T, 50/50; n log-uniform in
+    [nmin, nmax]; Erdos-Renyi mean nnz/row U[4,32] or banded half-bandwidth
+    U{2..40}; values U[-1,1)).  Batched = one graph launch per step over
+    the packed problems (tsg_batch); sequential = one plain tsg_spgemm call
+    per problem on the same resident tile operands."""
+    specs = ts.gen_batch_spec(count, seed=seed, nmin=nmin, nmax=nmax)
+    t0 = time.perf_counter()
+    mats = [ts.gen_batch_problem(sp) for sp in specs]
+    ops = [sp.op for sp in specs]
+    gen_s = time.perf_counter() - t0
+    progress(f"batch: generated {count} problems in {gen_s:.1f} s")
+    dps = [ctx.upload(m) for m in mats]
+    nnz_total = sum(m.nnz for m in mats)
+    del mats
+    # plain operands for the sequential calls
+    seq = []
+    for d, op in zip(dps, ops):
+        ta = ctx.csr2tile(d)
+        ctx.build_csc(ta)
+        if op == ts.OP_A2:
+            seq.append((ta, ta, None))
+        else:
+            dt = ctx.transpose(d)
+            tb = ctx.csr2tile(dt)
+            ctx.build_csc(tb)
+            ctx.free(dt)
+            seq.append((ta, tb, tb))
+    for ta, tb, _ in seq:  # warm the allocator cache
+        tc, _ = ctx.spgemm(ta, tb, timed=False)
+        ctx.free(tc)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for ta, tb, _ in seq:
+        tc, _ = ctx.spgemm(ta, tb, timed=False)
+        ctx.free(tc)
+    ctx.sync()
+    seq_s = time.perf_counter() - t0
+    progress(f"batch: sequential {seq_s * 1e3:.1f} ms")
+    for ta, tb, own in seq:
+        ctx.free(ta)
+        if own is not None:
+            ctx.free(own)
+    t0 = time.perf_counter()
+    b = ts.Batch(ctx, dps, ops)
+    create_s = time.perf_counter() - t0
+    progress(f"batch: created in {create_s:.1f} s")
+    tuned = None
+    if autotune:
+        t0 = time.perf_counter()
+        tt, best_ms = b.autotune(reps=2)
+        tuned = {"force_tile_parallel": tt.force_tile_parallel, "dense_acc_min_nnz": tt.dense_acc_min_nnz,
+                 "best_ms": round(best_ms, 3), "autotune_s": round(time.perf_counter() - t0, 2)}
+        progress(f"batch: autotuned {tuned}")
+    b.run()
+    ctx.sync()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        b.run()
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    info = b.info()
+    b.close()
+    for d in dps:
+        ctx.free(d)
+    step_s = statistics.mean(times)
+    return {"problems": count, "seed": seed, "nnz_total": nnz_total, "nnzC": info.nnzC, "tilesC": info.numtileC,
+            "groups": info.groups, "graph_nodes": info.graph_nodes,
+            "solves_per_s": round(count / step_s, 1), "ms_per_step": round(step_s * 1e3, 3),
+            "ms_per_step_min": round(min(times) * 1e3, 3),
+            "sequential_solves_per_s": round(count / seq_s, 1), "sequential_ms": round(seq_s * 1e3, 3),
+            "speedup_vs_sequential": round(seq_s / step_s, 2), "create_s": round(create_s, 2),
+            "generate_s": round(gen_s, 2), "autotune": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +153,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--dense-threshold", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run the plain copy kernel over this many bytes (byte-counter reference)")
@@ -84,6 +167,11 @@ def main():
     import numpy as np
     import tilespgemm as ts
 
+    if args.batch_only:
+        ctx = ts.Context(0)
+        print(json.dumps({"batched": batch_leg(ts, ctx, count=args.batch_count)}), flush=True)
+        ctx.close()
+        return
     N = args.grid
     a = ts.gen_stencil27(N, N, N, periodic=True, valmode=ts.VAL_UNIFORM_PM1, seed=1)
     ctx = ts.Context(0)
@@ -110,7 +198,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     ev_total = [s["t_total_ms"] for s in steps]
-    ev_s3 = [s["t_step3_ms"] for s in steps]
+    ev_s3 = [s["t_step3_rows_ms"] for s in steps]
     last = steps[-1]
     gflops = 2.0 * nnzcub / (ms_per_step * 1e6)
     s3_bytes = step3_algorithmic_bytes(ta, last["numtileC"], last["nnzC"], ta.tilen)
@@ -137,18 +225,22 @@ def main():
         "config": {"workload": f"27-point periodic stencil {N}^3, C = A^2", "n": a.m, "nnz": a.nnz,
                    "nnzCub": nnzcub, "nnzC": last["nnzC"], "tilesC": last["numtileC"],
                    "tilesC_empty": last["tiles_empty"], "values": "U[-1,1) seed 1"},
-        "roofline": {"bound": "hbm", "kernel": "k_step3 (numeric)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_step3_rows (numeric)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": s3_bytes, "avg_ms": round(s3_ms, 4)},
         "step_ms": {"step1": round(statistics.mean(s["t_step1_ms"] for s in steps), 4),
                     "step2": round(statistics.mean(s["t_step2_ms"] for s in steps), 4),
-                    "step3": round(s3_ms, 4),
+                    "step3": round(statistics.mean(s["t_step3_ms"] for s in steps), 4),
+                    "step3_rows_kernel": round(s3_ms, 4),
                     "event_total_min": round(min(ev_total), 4),
                     "event_total_median": round(statistics.median(ev_total), 4)},
         "gflops_event_min": round(2.0 * nnzcub / (min(ev_total) * 1e6), 2),
     }
     ctx.free(ta)
     ctx.free(da)
+    if not args.no_batch:
+        ctx.trim()
+        out["batched"] = batch_leg(ts, ctx, count=args.batch_count)
     ctx.close()
     if not args.no_cpu:
         rp = a.rowptr.astype(np.int64)

@@ -152,7 +152,9 @@ typedef struct {
     int force_tile_parallel; /* 1: steps 2 and 3 use the tile-parallel
                               kernels for every tile-row instead of the
                               row-wise LDS kernels (default 0 = by size) */
-    int reserved[5];
+    int batch_group_nnz_cap; /* batches: most C nonzeros per fused group
+                              (0 = the int32 limit of the tile offsets) */
+    int reserved[4];
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 
@@ -160,6 +162,7 @@ typedef struct {
     double t_step1_ms, t_step2_ms, t_step3_ms, t_alloc_ms, t_total_ms;
     long long numtileC, nnzC;
     long long tiles_sparse_acc, tiles_dense_acc, tiles_empty;
+    double t_step3_rows_ms; /* the row-wise numeric kernel's launch alone */
 } tsg_stats;
 
 /* tune and stats may be NULL.  When stats is given, each step is bracketed
@@ -217,6 +220,56 @@ int tsg_dcsr_symmetrize(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dcsr *g);
  * src/reorder.h:2506-2551. */
 int tsg_dcsr_permute(tsg_ctx *ctx, const tsg_dcsr *a, const int *d_row_old, const int *d_col_new,
                      tsg_dcsr *out);
+
+/* ------------------------------------------------------------------ */
+/* Batched SpGEMM: many independent products per step.  There is no
+ * counterpart in the reference (one product per tilespgemm() call,
+ * src/tilespgemm-cuda.h:2610); each product is the one tilespgemm()
+ * computes for the driver's A^2 or AA^T mode (src/main.cu:199-241), and
+ * each result is bit-identical to tsg_spgemm on the same operands.
+ * The problems are packed block-diagonally on 16-row/column boundaries so
+ * every pipeline kernel runs once for the whole batch; the pipeline is
+ * planned once and captured in a HIP graph, and tsg_batch_run is one
+ * graph launch recomputing every product into resident C buffers.      */
+enum { TSG_OP_A2 = 0, TSG_OP_AAT = 1 };
+typedef struct tsg_batch tsg_batch;
+typedef struct {
+    int count;            /* problems */
+    int groups;           /* fused pipelines (split where C overflows int32) */
+    long long numtileC;   /* C tiles over all problems */
+    long long nnzC;       /* C nonzeros over all problems */
+    int graph_nodes;
+    double build_ms;      /* packing, tile conversion and planning */
+    double capture_ms;    /* graph capture + instantiation */
+    tsg_tune tune;        /* configuration in use */
+} tsg_batch_info;
+/* problems[i] are device CSRs (read during create only; the batch keeps
+ * its own packed copy); ops[i] is TSG_OP_A2 (square) or TSG_OP_AAT. */
+int tsg_batch_create(tsg_ctx *ctx, const tsg_dcsr *problems, const int *ops, int count,
+                     const tsg_tune *tune, tsg_batch **out);
+/* One step: enqueue the captured graph on the context stream (async). */
+int tsg_batch_run(tsg_batch *b);
+/* Times every candidate configuration over `reps` graph launches and
+ * keeps the fastest (chosen / best_ms may be NULL). */
+int tsg_batch_autotune(tsg_batch *b, int reps, tsg_tune *chosen, double *best_ms);
+int tsg_batch_get_info(const tsg_batch *b, tsg_batch_info *out);
+/* Problem i's C (after a run): tile arrays in the problem's own indices
+ * and/or CSR; either pointer may be NULL.  Synchronizes the stream. */
+int tsg_batch_result(tsg_batch *b, int i, tsg_htiles *tiles, tsg_hcsr *csr);
+void tsg_batch_destroy(tsg_batch *b);
+
+/* The batched workload: `count` problems from `seed`; n log-uniform in
+ * [nmin, nmax]; family 0 = Erdos-Renyi with mean nnz/row U[4,32], 1 =
+ * banded with half-bandwidth uniform in {2..40}; op A^2 or AA^T with
+ * probability 1/2 each; seed_i for the generator of problem i. */
+typedef struct {
+    int n, family, op, halfbw;
+    double mean_row;
+    unsigned long long seed;
+} tsg_batch_spec;
+int tsg_gen_batch_spec(int count, unsigned long long seed, int nmin, int nmax, tsg_batch_spec *out);
+/* Generates problem `s` (values U[-1,1)). */
+int tsg_gen_batch_problem(const tsg_batch_spec *s, tsg_hcsr *out);
 
 /* Library build information (compile flags, target). */
 const char *tsg_build_info(void);

@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(256)
 k_t2c_rows(int m, const int *__restrict__ tile_ptr, const int *__restrict__ tile_colidx,
            const int *__restrict__ tile_nnz, const unsigned char *__restrict__ csr_ptr,
            const unsigned char *__restrict__ tcol, const double *__restrict__ tval,
-           int *__restrict__ rowcnt_or_ptr, int *__restrict__ col, double *__restrict__ val) {
+           int *__restrict__ rowcnt_or_ptr, int *__restrict__ col, double *__restrict__ val, int jbase) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     const int I = i >> 4, r = i & 15;
@@ -334,7 +334,7 @@ k_t2c_rows(int m, const int *__restrict__ tile_ptr, const int *__restrict__ tile
             const int k = t2c_row_len(tile_nnz, csr_ptr, t, r, rowlen);
             if (k <= 0) continue;
             const int s = tile_nnz[t] + csr_ptr[(long long)t * 16 + r];
-            const int J16 = tile_colidx[t] * 16;
+            const int J16 = (tile_colidx[t] - jbase) * 16;
             for (int j = 0; j < k; j++) {
                 col[out + j] = J16 + tcol[s + j];
                 val[out + j] = tval[s + j];
@@ -345,6 +345,13 @@ k_t2c_rows(int m, const int *__restrict__ tile_ptr, const int *__restrict__ tile
 }
 
 extern "C" int tsg_tile2csr(tsg_ctx *ctx, const tsg_dtiles *t, tsg_dcsr *out) {
+    return tsg_tile2csr_view(ctx, t, 0, out);
+}
+
+// t may be a view of a tile-row range of a larger tile matrix (tile_ptr
+// pointing into the parent's, every other array the parent's, indices
+// absolute); output columns are taken relative to tile column jbase.
+int tsg_tile2csr_view(tsg_ctx *ctx, const tsg_dtiles *t, int jbase, tsg_dcsr *out) {
     memset(out, 0, sizeof(*out));
     out->m = t->m;
     out->n = t->n;
@@ -352,7 +359,7 @@ extern "C" int tsg_tile2csr(tsg_ctx *ctx, const tsg_dtiles *t, tsg_dcsr *out) {
     if (t->m > 0) {
         hipLaunchKernelGGL(k_t2c_rows<0>, dim3(tsg_div_up(t->m, 256)), dim3(256), 0, ctx->stream, t->m, t->tile_ptr,
                            t->tile_colidx, t->tile_nnz, t->csr_ptr, t->csr_col, t->val, out->rowptr,
-                           (int *)nullptr, (double *)nullptr);
+                           (int *)nullptr, (double *)nullptr, jbase);
         TSG_LAUNCH_CHECK();
     }
     long long nnz = 0;
@@ -363,7 +370,7 @@ extern "C" int tsg_tile2csr(tsg_ctx *ctx, const tsg_dtiles *t, tsg_dcsr *out) {
     if (t->m > 0 && nnz > 0) {
         hipLaunchKernelGGL(k_t2c_rows<1>, dim3(tsg_div_up(t->m, 256)), dim3(256), 0, ctx->stream, t->m, t->tile_ptr,
                            t->tile_colidx, t->tile_nnz, t->csr_ptr, t->csr_col, t->val, out->rowptr,
-                           out->colidx, out->val);
+                           out->colidx, out->val, jbase);
         TSG_LAUNCH_CHECK();
     }
     return TSG_OK;

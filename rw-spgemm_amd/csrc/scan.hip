@@ -89,29 +89,37 @@ __global__ void k_copy_total_i32(const long long *partial, long long nb, int *ou
     out[n] = t > 0x7fffffffLL ? 0x7fffffff : (int)t;
 }
 
-int tsg_scan_i32(tsg_ctx *ctx, const int *in, int *out, long long n, long long *total) {
+long long tsg_scan_partial_count(long long n) {
+    return (n <= 0 ? 1 : (n + SCAN_TILE - 1) / SCAN_TILE) + 1;
+}
+
+int tsg_scan_i32_enqueue(hipStream_t stream, const int *in, int *out, long long n, long long *partial) {
     if (n < 0) { tsg_set_error("tsg_scan_i32: n < 0"); return TSG_ERR_ARG; }
-    long long nb = n == 0 ? 1 : (n + SCAN_TILE - 1) / SCAN_TILE;
-    void *scr;
-    TSG_TRY(tsg_scratch(ctx, (size_t)(nb + 1) * sizeof(long long), &scr));
-    long long *partial = (long long *)scr;
-    if (n > 0) {
-        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, partial);
-        TSG_LAUNCH_CHECK();
-    } else {
-        TSG_HIP(hipMemsetAsync(partial, 0, sizeof(long long), ctx->stream));
-    }
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, partial, nb);
+    const long long nb = tsg_scan_partial_count(n) - 1;
+    // with n == 0 the single reduce block writes partial[0] = 0
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, stream, in, n, partial);
+    TSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, partial, nb);
     TSG_LAUNCH_CHECK();
     if (n > 0) {
-        hipLaunchKernelGGL(k_scan_down<int>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n,
+        hipLaunchKernelGGL(k_scan_down<int>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, stream, in, n,
                            (const long long *)partial, out);
         TSG_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_copy_total_i32, dim3(1), dim3(1), 0, ctx->stream, (const long long *)partial, nb, out, n);
+    hipLaunchKernelGGL(k_copy_total_i32, dim3(1), dim3(1), 0, stream, (const long long *)partial, nb, out, n);
     TSG_LAUNCH_CHECK();
+    return TSG_OK;
+}
+
+int tsg_scan_i32(tsg_ctx *ctx, const int *in, int *out, long long n, long long *total) {
+    if (n < 0) { tsg_set_error("tsg_scan_i32: n < 0"); return TSG_ERR_ARG; }
+    const long long np = tsg_scan_partial_count(n);
+    void *scr;
+    TSG_TRY(tsg_scratch(ctx, (size_t)np * sizeof(long long), &scr));
+    long long *partial = (long long *)scr;
+    TSG_TRY(tsg_scan_i32_enqueue(ctx->stream, in, out, n, partial));
     if (total) {
-        TSG_TRY(tsg_read_scalar_i64(ctx, partial + nb, 1, total));
+        TSG_TRY(tsg_read_scalar_i64(ctx, partial + np - 1, 1, total));
         if (*total > 0x7fffffffLL) {
             tsg_set_error("prefix sum total %lld exceeds int32", *total);
             return TSG_ERR_OVERFLOW;

@@ -854,8 +854,93 @@ static int ev_ms(hipEvent_t a, hipEvent_t b, double *ms) {
     return TSG_OK;
 }
 
-extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
-                          tsg_dtiles *C, tsg_stats *stats) {
+// Launch helpers shared by the synchronous pipeline and the enqueue-only
+// (graph-capturable) one.
+static TileOperands make_operands(const tsg_dtiles *A, const tsg_dtiles *B) {
+    TileOperands op;
+    op.Aptr = A->tile_ptr; op.Acol = A->tile_colidx; op.Annz = A->tile_nnz; op.Aptr16 = A->csr_ptr;
+    op.Amask = A->mask; op.Aval = A->val;
+    op.Bptr = B->tile_ptr; op.Bcol = B->tile_colidx;
+    op.Bcscptr = B->csc_ptr; op.Bcscrow = B->csc_rowidx; op.Bcscid = B->csc_tileid; op.Bnnz = B->tile_nnz;
+    op.Bptr16 = B->csr_ptr; op.Bmask = B->mask; op.Bval = B->val;
+    return op;
+}
+
+static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C, bool fill) {
+    if (C->tilem <= 0) return TSG_OK;
+    if (!fill) {
+        hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0, s,
+                           A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, C->tile_ptr,
+                           (int *)nullptr, (int *)nullptr);
+    } else {
+        hipLaunchKernelGGL((k_step1<true, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0, s,
+                           A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, C->tile_ptr,
+                           C->tile_colidx, C->tile_rowidx);
+    }
+    TSG_LAUNCH_CHECK();
+    return TSG_OK;
+}
+
+static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P) {
+    if (C->numtile <= 0) return TSG_OK;
+    if (!P.all) {
+        hipLaunchKernelGGL(k_step2_rows, dim3(C->tilem), dim3(G2_THREADS), 0, s, op, C->tile_ptr, C->tile_colidx,
+                           C->tilem, C->mask, C->csr_ptr, C->tile_nnz);
+        TSG_LAUNCH_CHECK();
+    }
+    if (P.all || P.maxrow > G2_UMAX) {
+        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
+        hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
+                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, P.all);
+        TSG_LAUNCH_CHECK();
+    }
+    return TSG_OK;
+}
+
+static int launch_row_class(hipStream_t s, const TileOperands &op, tsg_dtiles *C, unsigned char *rowcls,
+                            int *cls_out) {
+    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
+                       C->tilem, rowcls, cls_out);
+    TSG_LAUNCH_CHECK();
+    return TSG_OK;
+}
+
+static int launch_step3(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P,
+                        const unsigned char *rowcls, hipEvent_t *ev = nullptr) {
+    if (C->nnz <= 0) return TSG_OK;
+    if (!P.all && P.s3_rows) {
+        if (ev) TSG_HIP(hipEventRecord(ev[0], s));
+        G3Layout L;
+        L.umax = P.g3[0]; L.nnzmax = P.g3[1]; L.lamax = P.g3[2];
+        L.qb = P.g3[3]; L.bnnz = P.g3[4]; L.amax = P.g3[5];
+        size_t off[G3_NARR];
+        const size_t lds = g3_layout(L, off);
+        hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), lds, s, op, C->tile_ptr, C->tile_colidx,
+                           C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col, C->val, L);
+        TSG_LAUNCH_CHECK();
+        if (ev) TSG_HIP(hipEventRecord(ev[1], s));
+    }
+    if (P.all || P.s3_unfit) {
+        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
+        hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
+                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val,
+                           P.dense_min, P.all, rowcls);
+        TSG_LAUNCH_CHECK();
+    }
+    return TSG_OK;
+}
+
+static int alloc_c_tiles(tsg_ctx *ctx, tsg_dtiles *C, long long numtileC) {
+    TSG_TRY(tsg_alloc_t(ctx, &C->tile_colidx, (size_t)numtileC + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &C->tile_rowidx, (size_t)numtileC + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &C->mask, (size_t)numtileC * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &C->csr_ptr, (size_t)numtileC * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &C->tile_nnz, (size_t)numtileC + 1));
+    return TSG_OK;
+}
+
+int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
+                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out) {
     memset(C, 0, sizeof(*C));
     if (!A || !B || A->n != B->m) {
         tsg_set_error("tsg_spgemm: inner dimensions differ (%d vs %d)", A ? A->n : -1, B ? B->m : -1);
@@ -865,13 +950,20 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     tsg_tune tune;
     if (tune_in) tune = *tune_in; else tsg_tune_default(&tune);
     if (tune.dense_acc_min_nnz < 1) tune.dense_acc_min_nnz = 1;
+    tsg_spgemm_plan P;
+    memset(&P, 0, sizeof(P));
+    P.all = tune.force_tile_parallel ? 1 : 0;
+    P.dense_min = tune.dense_acc_min_nnz;
+    hipStream_t s = ctx->stream;
 
-    hipEvent_t ev[4];
+    hipEvent_t ev[6];
     const bool timed = stats != nullptr;
     if (timed) {
         memset(stats, 0, sizeof(*stats));
         for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
-        TSG_HIP(hipEventRecord(ev[0], ctx->stream));
+        TSG_HIP(hipEventRecord(ev[4], s));
+        TSG_HIP(hipEventRecord(ev[5], s));
+        TSG_HIP(hipEventRecord(ev[0], s));
     }
     C->m = A->m;
     C->n = B->n;
@@ -879,122 +971,73 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     C->tilen = B->tilen;
     // ---- step 1
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_ptr, (size_t)C->tilem + 1));
-    if (C->tilem > 0) {
-        hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
-                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx,
-                           C->tile_ptr, (int *)nullptr, (int *)nullptr);
-        TSG_LAUNCH_CHECK();
-    }
-    long long numtileC = 0;
-    TSG_TRY(tsg_scan_i32(ctx, C->tile_ptr, C->tile_ptr, C->tilem, &numtileC));
-    C->numtile = (int)numtileC;
-    TSG_TRY(tsg_alloc_t(ctx, &C->tile_colidx, (size_t)numtileC + 1));
-    TSG_TRY(tsg_alloc_t(ctx, &C->tile_rowidx, (size_t)numtileC + 1));
+    TSG_TRY(launch_step1(s, A, B, C, false));
     void *scr;
     TSG_TRY(tsg_alloc(ctx, &scr, 64));
     unsigned long long *bins = (unsigned long long *)scr;
     int *maxrow = (int *)(bins + 4);
-    TSG_HIP(hipMemsetAsync(scr, 0, 64, ctx->stream));
+    TSG_HIP(hipMemsetAsync(scr, 0, 64, s));
+    long long numtileC = 0;
+    TSG_TRY(tsg_scan_i32(ctx, C->tile_ptr, C->tile_ptr, C->tilem, &numtileC));
+    P.numtileC = numtileC;
+    C->numtile = (int)numtileC;
+    TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
+    TSG_TRY(launch_step1(s, A, B, C, true));
     if (numtileC > 0) {
-        hipLaunchKernelGGL((k_step1<true, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
-                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx,
-                           C->tile_ptr, C->tile_colidx, C->tile_rowidx);
+        hipLaunchKernelGGL(k_max_rowlen, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))), dim3(256), 0, s,
+                           C->tile_ptr, C->tilem, maxrow);
         TSG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_max_rowlen, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))), dim3(256), 0,
-                           ctx->stream, C->tile_ptr, C->tilem, maxrow);
-        TSG_LAUNCH_CHECK();
-    }
-    if (timed) TSG_HIP(hipEventRecord(ev[1], ctx->stream));
-    // ---- step 2
-    TSG_TRY(tsg_alloc_t(ctx, &C->mask, (size_t)numtileC * 16 + 16));
-    TSG_TRY(tsg_alloc_t(ctx, &C->csr_ptr, (size_t)numtileC * 16 + 16));
-    TSG_TRY(tsg_alloc_t(ctx, &C->tile_nnz, (size_t)numtileC + 1));
-    TileOperands op;
-    op.Aptr = A->tile_ptr; op.Acol = A->tile_colidx; op.Annz = A->tile_nnz; op.Aptr16 = A->csr_ptr;
-    op.Amask = A->mask; op.Aval = A->val;
-    op.Bptr = B->tile_ptr; op.Bcol = B->tile_colidx;
-    op.Bcscptr = B->csc_ptr; op.Bcscrow = B->csc_rowidx; op.Bcscid = B->csc_tileid; op.Bnnz = B->tile_nnz;
-    op.Bptr16 = B->csr_ptr; op.Bmask = B->mask; op.Bval = B->val;
-    const unsigned tiles_per_block = S23_WAVES * TILES_PER_WAVE;
-    const unsigned nblk = tsg_div_up(numtileC, tiles_per_block);
-    if (numtileC > 0) {
         long long mx = 0;
         TSG_TRY(tsg_read_scalar_i64(ctx, maxrow, 0, &mx));
-        const int all = tune.force_tile_parallel ? 1 : 0;
-        if (!all) {
-            hipLaunchKernelGGL(k_step2_rows, dim3(C->tilem), dim3(G2_THREADS), 0, ctx->stream, op, C->tile_ptr,
-                               C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz);
-            TSG_LAUNCH_CHECK();
-        }
-        if (all || mx > G2_UMAX) {
-            hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
-                               C->tile_rowidx, C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, all);
-            TSG_LAUNCH_CHECK();
-        }
-        if (timed) {
-            hipLaunchKernelGGL(k_bins, dim3(std::min<unsigned>(2048, tsg_div_up(numtileC, 256))), dim3(256), 0,
-                               ctx->stream, C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
-            TSG_LAUNCH_CHECK();
-        }
+        P.maxrow = (int)mx;
+    }
+    if (timed) TSG_HIP(hipEventRecord(ev[1], s));
+    // ---- step 2
+    const TileOperands op = make_operands(A, B);
+    TSG_TRY(launch_step2(s, op, C, P));
+    if (timed && numtileC > 0) {
+        hipLaunchKernelGGL(k_bins, dim3(std::min<unsigned>(2048, tsg_div_up(numtileC, 256))), dim3(256), 0, s,
+                           C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
+        TSG_LAUNCH_CHECK();
     }
     long long nnzC = 0;
     TSG_TRY(tsg_scan_i32(ctx, C->tile_nnz, C->tile_nnz, numtileC, &nnzC));
+    P.nnzC = nnzC;
     C->nnz = (int)nnzC;
-    if (timed) TSG_HIP(hipEventRecord(ev[2], ctx->stream));
+    if (timed) TSG_HIP(hipEventRecord(ev[2], s));
     // ---- step 3
     TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
     TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
     if (nnzC > 0) {
-        const int all = tune.force_tile_parallel ? 1 : 0;
-        long long unfit = 1;
         unsigned char *rowcls = nullptr;
         int *cls_out = nullptr;
         TSG_TRY(tsg_alloc_t(ctx, &rowcls, (size_t)C->tilem + 1));
         TSG_TRY(tsg_alloc_t(ctx, &cls_out, 8));
-        if (!all) {
-            TSG_HIP(hipMemsetAsync(cls_out, 0, 8 * sizeof(int), ctx->stream));
-            hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, ctx->stream, op,
-                               C->tile_ptr, C->tile_nnz, C->tilem, rowcls, cls_out);
-            TSG_LAUNCH_CHECK();
+        if (!P.all) {
+            TSG_HIP(hipMemsetAsync(cls_out, 0, 8 * sizeof(int), s));
+            TSG_TRY(launch_row_class(s, op, C, rowcls, cls_out));
             int h[8];
-            TSG_HIP(hipMemcpyAsync(h, cls_out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-            TSG_HIP(hipStreamSynchronize(ctx->stream));
-            unfit = h[0];
-            G3Layout L;
-            L.umax = std::max(1, h[1]);
-            L.nnzmax = std::max(1, h[2]);
-            L.lamax = std::max(1, h[3]);
-            L.qb = std::max(1, h[4]);
-            L.bnnz = std::max(1, h[5]);
-            L.amax = std::max(1, h[6]);
-            size_t off[G3_NARR];
-            const size_t lds = g3_layout(L, off);
-            if (h[1] > 0) {
-                hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), lds, ctx->stream, op, C->tile_ptr,
-                                   C->tile_colidx, C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col,
-                                   C->val, L);
-                TSG_LAUNCH_CHECK();
-            }
+            TSG_HIP(hipMemcpyAsync(h, cls_out, sizeof(h), hipMemcpyDeviceToHost, s));
+            TSG_HIP(hipStreamSynchronize(s));
+            P.s3_unfit = h[0] > 0;
+            P.s3_rows = h[1] > 0;
+            for (int i = 0; i < 6; i++) P.g3[i] = std::max(1, h[1 + i]);
         }
-        if (unfit > 0) {
-            hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, ctx->stream, op, C->tile_ptr,
-                               C->tile_rowidx, C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz,
-                               C->csr_col, C->val, tune.dense_acc_min_nnz, all, (const unsigned char *)rowcls);
-            TSG_LAUNCH_CHECK();
-        }
+        TSG_TRY(launch_step3(s, op, C, P, rowcls, timed ? ev + 4 : nullptr));
         tsg_free(ctx, rowcls);
         tsg_free(ctx, cls_out);
     }
     if (timed) {
-        TSG_HIP(hipEventRecord(ev[3], ctx->stream));
+        TSG_HIP(hipEventRecord(ev[3], s));
         TSG_HIP(hipEventSynchronize(ev[3]));
         TSG_TRY(ev_ms(ev[0], ev[1], &stats->t_step1_ms));
         TSG_TRY(ev_ms(ev[1], ev[2], &stats->t_step2_ms));
         TSG_TRY(ev_ms(ev[2], ev[3], &stats->t_step3_ms));
         TSG_TRY(ev_ms(ev[0], ev[3], &stats->t_total_ms));
+        TSG_TRY(ev_ms(ev[4], ev[5], &stats->t_step3_rows_ms));
         unsigned long long hb[4];
-        TSG_HIP(hipMemcpyAsync(hb, bins, sizeof(hb), hipMemcpyDeviceToHost, ctx->stream));
-        TSG_HIP(hipStreamSynchronize(ctx->stream));
+        TSG_HIP(hipMemcpyAsync(hb, bins, sizeof(hb), hipMemcpyDeviceToHost, s));
+        TSG_HIP(hipStreamSynchronize(s));
         stats->tiles_empty = (long long)hb[0];
         stats->tiles_sparse_acc = (long long)hb[1];
         stats->tiles_dense_acc = (long long)hb[2];
@@ -1003,5 +1046,35 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
         for (auto &e : ev) hipEventDestroy(e);
     }
     tsg_free(ctx, scr);
+    if (plan_out) *plan_out = P;
+    return TSG_OK;
+}
+
+extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
+                          tsg_dtiles *C, tsg_stats *stats) {
+    return tsg_spgemm_planned(ctx, A, B, tune_in, C, stats, nullptr);
+}
+
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C) {
+    return ((size_t)tsg_scan_partial_count(C->tilem) + (size_t)tsg_scan_partial_count(C->numtile)) * 8 +
+           (((size_t)C->tilem + 1 + 15) & ~(size_t)15) + 64;
+}
+
+int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
+                       const tsg_spgemm_plan &P, void *work) {
+    long long *part1 = (long long *)work;
+    long long *part2 = part1 + tsg_scan_partial_count(C->tilem);
+    unsigned char *rowcls = (unsigned char *)(part2 + tsg_scan_partial_count(C->numtile));
+    int *cls_out = (int *)(rowcls + (((size_t)C->tilem + 1 + 15) & ~(size_t)15));
+    TSG_TRY(launch_step1(s, A, B, C, false));
+    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, part1));
+    TSG_TRY(launch_step1(s, A, B, C, true));
+    const TileOperands op = make_operands(A, B);
+    TSG_TRY(launch_step2(s, op, C, P));
+    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, part2));
+    if (C->nnz > 0) {
+        if (!P.all) TSG_TRY(launch_row_class(s, op, C, rowcls, cls_out));
+        TSG_TRY(launch_step3(s, op, C, P, rowcls));
+    }
     return TSG_OK;
 }

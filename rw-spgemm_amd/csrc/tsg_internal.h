@@ -74,10 +74,41 @@ int tsg_read_scalar_i64(tsg_ctx *ctx, const void *dptr, int is64, long long *out
 // total).  `in` may alias `out`.  total (may be NULL) is read back to the
 // host; TSG_ERR_OVERFLOW when the total exceeds INT32_MAX.
 int tsg_scan_i32(tsg_ctx *ctx, const int *in, int *out, long long n, long long *total);
+// Enqueue-only form (no host read-back, no memset: graph-capturable).
+// `partial` holds tsg_scan_partial_count(n) int64 values; the int64 total
+// is left in partial[tsg_scan_partial_count(n) - 1].
+long long tsg_scan_partial_count(long long n);
+int tsg_scan_i32_enqueue(hipStream_t stream, const int *in, int *out, long long n, long long *partial);
 // Same for unsigned-char counts of 16-row tiles is done inside the kernels.
 // Sum of n int64 values (device) -> host.
 int tsg_reduce_u64(tsg_ctx *ctx, const unsigned long long *in, long long n,
                    unsigned long long *out);
+
+// ------------------------------------------------------------------ spgemm
+// Decisions of one SpGEMM that depend only on the operand patterns (sizes
+// of C, which step-2 / step-3 kernels run, the LDS layout of the row-wise
+// step 3).  A plan made by tsg_spgemm_planned lets tsg_spgemm_enqueue
+// recompute the same product into the same C buffers with no host
+// read-back, allocation or memset (so the sequence can be captured in a
+// HIP graph).
+struct tsg_spgemm_plan {
+    long long numtileC, nnzC;
+    int maxrow;     // C tiles of the longest tile-row
+    int all;        // every tile-row through the tile-parallel kernels
+    int s3_rows;    // some tile-row runs the row-wise step 3
+    int s3_unfit;   // some tile-row runs the tile-parallel step 3
+    int dense_min;  // dense accumulator threshold of the tile-parallel step 3
+    int g3[6];      // row-wise step-3 LDS layout maxima
+};
+int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune,
+                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out);
+// Device scratch bytes tsg_spgemm_enqueue needs for this C.
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C);
+int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
+                       const tsg_spgemm_plan &P, void *work);
+
+// ------------------------------------------------------------------ conversions
+int tsg_tile2csr_view(tsg_ctx *ctx, const tsg_dtiles *t, int jbase, tsg_dcsr *out);
 
 // ------------------------------------------------------------------ sort
 // Stable ascending sort of (key,value) pairs, 32-bit keys, using the

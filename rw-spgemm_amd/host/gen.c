@@ -254,3 +254,29 @@ int tsg_gen_rmat(int scale, int edge_factor, double a, double b, double c, int s
     free(keys);
     return tsg_hcsr_set_values(out, valmode == TSG_VAL_KEEP ? TSG_VAL_UNIFORM_01 : valmode, seed);
 }
+
+/* The batched workload (see tsg.h): draws of problem i come from stream
+ * 100 of the batch seed, five per problem. */
+int tsg_gen_batch_spec(int count, unsigned long long seed, int nmin, int nmax, tsg_batch_spec *out) {
+    if (count < 0 || nmin < 1 || nmax < nmin || !out) return TSG_ERR_ARG;
+    const double l0 = log((double)nmin), l1 = log((double)nmax);
+    for (int i = 0; i < count; i++) {
+        const uint64_t b = 5 * (uint64_t)i;
+        tsg_batch_spec *s = &out[i];
+        int n = (int)llround(exp(l0 + (l1 - l0) * u01(rng(seed, 100, b))));
+        s->n = n < nmin ? nmin : (n > nmax ? nmax : n);
+        s->family = u01(rng(seed, 100, b + 1)) < 0.5 ? 0 : 1;
+        s->op = u01(rng(seed, 100, b + 2)) < 0.5 ? TSG_OP_A2 : TSG_OP_AAT;
+        const double u = u01(rng(seed, 100, b + 3));
+        s->mean_row = 4.0 + 28.0 * u;
+        s->halfbw = 2 + (int)(39.0 * u);
+        s->seed = rng(seed, 101, (uint64_t)i);
+    }
+    return TSG_OK;
+}
+
+int tsg_gen_batch_problem(const tsg_batch_spec *s, tsg_hcsr *out) {
+    if (!s) return TSG_ERR_ARG;
+    if (s->family == 0) return tsg_gen_er(s->n, s->mean_row, TSG_VAL_UNIFORM_PM1, s->seed, out);
+    return tsg_gen_banded(s->n, s->halfbw, TSG_VAL_UNIFORM_PM1, s->seed, out);
+}

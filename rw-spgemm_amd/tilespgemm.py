@@ -77,13 +77,13 @@ class HTiles(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("force_tile_parallel", _i),
-                ("reserved", _i * 5)]
+                ("batch_group_nnz_cap", _i), ("reserved", _i * 4)]
 
 
 class Stats(C.Structure):
     _fields_ = [("t_step1_ms", _d), ("t_step2_ms", _d), ("t_step3_ms", _d), ("t_alloc_ms", _d),
                 ("t_total_ms", _d), ("numtileC", _ll), ("nnzC", _ll), ("tiles_sparse_acc", _ll),
-                ("tiles_dense_acc", _ll), ("tiles_empty", _ll)]
+                ("tiles_dense_acc", _ll), ("tiles_empty", _ll), ("t_step3_rows_ms", _d)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -98,6 +98,18 @@ class SMatrix(C.Structure):
                 ("tile_csr_Value", C.POINTER(_d)), ("tile_csr_Col", C.POINTER(C.c_ubyte)),
                 ("tile_csr_Ptr", C.POINTER(C.c_ubyte)), ("mask", C.POINTER(C.c_ushort)),
                 ("csc_tile_ptr", C.POINTER(_i)), ("csc_tile_rowidx", C.POINTER(_i))]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", _i), ("groups", _i), ("numtileC", _ll), ("nnzC", _ll), ("graph_nodes", _i),
+                ("build_ms", _d), ("capture_ms", _d), ("tune", Tune)]
+
+
+class BatchSpec(C.Structure):
+    _fields_ = [("n", _i), ("family", _i), ("op", _i), ("halfbw", _i), ("mean_row", _d), ("seed", _ull)]
+
+
+OP_A2, OP_AAT = 0, 1
 
 
 def _sig(name, res, args):
@@ -165,6 +177,14 @@ _sig("tilespgemm", None, [C.POINTER(SMatrix), C.POINTER(SMatrix), C.POINTER(SMat
                           _ull, C.POINTER(_ull), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.c_char_p,
                           C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)])
 _sig("tsg_nnzcub", _ull, [C.POINTER(SMatrix), C.POINTER(SMatrix)])
+_sig("tsg_batch_create", _i, [_p, C.POINTER(DCsr), C.POINTER(_i), _i, C.POINTER(Tune), C.POINTER(_p)])
+_sig("tsg_batch_run", _i, [_p])
+_sig("tsg_batch_autotune", _i, [_p, _i, C.POINTER(Tune), C.POINTER(_d)])
+_sig("tsg_batch_get_info", _i, [_p, C.POINTER(BatchInfo)])
+_sig("tsg_batch_result", _i, [_p, _i, C.POINTER(HTiles), C.POINTER(HCsr)])
+_sig("tsg_batch_destroy", None, [_p])
+_sig("tsg_gen_batch_spec", _i, [_i, _ull, _i, _i, C.POINTER(BatchSpec)])
+_sig("tsg_gen_batch_problem", _i, [C.POINTER(BatchSpec), C.POINTER(HCsr)])
 _sig("tsg_compat_set_device", _i, [_i])
 
 
@@ -222,6 +242,83 @@ def _take_hcsr(h):
         return Csr(m, n, rp, ci, va)
     finally:
         lib.tsg_hcsr_free(C.byref(h))
+
+
+def _take_htiles(h) -> dict:
+    """Copy a library-allocated HTiles into numpy arrays and free it."""
+    try:
+        nt = h.numtile
+        out = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, numtile=nt, nnz=h.nnz)
+        out["tile_ptr"] = _arr(h.tile_ptr, np.int32, h.tilem + 1)
+        out["tile_colidx"] = _arr(h.tile_colidx, np.int32, nt)
+        out["tile_rowidx"] = _arr(h.tile_rowidx, np.int32, nt)
+        out["tile_nnz"] = _arr(h.tile_nnz, np.int32, nt + 1)
+        out["csr_ptr"] = _arr(h.csr_ptr, np.uint8, nt * 16)
+        out["csr_col"] = _arr(h.csr_col, np.uint8, h.nnz)
+        out["val"] = _arr(h.val, np.float64, h.nnz)
+        out["mask"] = _arr(h.mask, np.uint16, nt * 16)
+        out["csc_ptr"] = _arr(h.csc_ptr, np.int32, h.tilen + 1) if h.csc_ptr else None
+        out["csc_rowidx"] = _arr(h.csc_rowidx, np.int32, nt) if h.csc_rowidx else None
+        out["csc_tileid"] = _arr(h.csc_tileid, np.int32, nt) if h.csc_tileid else None
+        return out
+    finally:
+        lib.tsg_htiles_free(C.byref(h))
+
+
+def gen_batch_spec(count, seed=2, nmin=1000, nmax=65536):
+    """The batched workload's problem list (tsg_gen_batch_spec)."""
+    arr = (BatchSpec * max(1, count))()
+    _check(lib.tsg_gen_batch_spec(count, seed, nmin, nmax, arr))
+    return [arr[i] for i in range(count)]
+
+
+def gen_batch_problem(spec: BatchSpec) -> Csr:
+    h = HCsr()
+    _check(lib.tsg_gen_batch_problem(C.byref(spec), C.byref(h)))
+    return _take_hcsr(h)
+
+
+class Batch:
+    """Many SpGEMMs per step (tsg_batch): packed block-diagonally, planned
+    once, one HIP graph launch per run."""
+
+    def __init__(self, ctx: "Context", problems, ops, tune: Tune | None = None):
+        self.ctx = ctx
+        n = len(problems)
+        arr = (DCsr * n)(*problems)
+        o = (_i * n)(*[int(x) for x in ops])
+        p = _p()
+        _check(lib.tsg_batch_create(ctx.h, arr, o, n, C.byref(tune) if tune else None, C.byref(p)))
+        self.h = p
+
+    def run(self):
+        _check(lib.tsg_batch_run(self.h))
+
+    def autotune(self, reps=3):
+        t = Tune()
+        ms = _d()
+        _check(lib.tsg_batch_autotune(self.h, reps, C.byref(t), C.byref(ms)))
+        return t, ms.value
+
+    def info(self) -> BatchInfo:
+        bi = BatchInfo()
+        _check(lib.tsg_batch_get_info(self.h, C.byref(bi)))
+        return bi
+
+    def result_tiles(self, i) -> dict:
+        h = HTiles()
+        _check(lib.tsg_batch_result(self.h, i, C.byref(h), None))
+        return _take_htiles(h)
+
+    def result_csr(self, i) -> "Csr":
+        h = HCsr()
+        _check(lib.tsg_batch_result(self.h, i, None, C.byref(h)))
+        return _take_hcsr(h)
+
+    def close(self):
+        if self.h:
+            lib.tsg_batch_destroy(self.h)
+            self.h = None
 
 
 def gen_stencil5(nx, ny):
@@ -297,6 +394,10 @@ class Context:
     def sync(self):
         _check(lib.tsg_ctx_sync(self.h))
 
+    def trim(self):
+        """Return the allocator's cached free blocks to the device."""
+        lib.tsg_ctx_trim(self.h)
+
     def bytes_in_use(self):
         return lib.tsg_ctx_bytes_in_use(self.h)
 
@@ -368,23 +469,7 @@ class Context:
     def download_tiles(self, t: DTiles) -> dict:
         h = HTiles()
         _check(lib.tsg_tiles_download(self.h, C.byref(t), C.byref(h)))
-        try:
-            nt = h.numtile
-            out = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, numtile=nt, nnz=h.nnz)
-            out["tile_ptr"] = _arr(h.tile_ptr, np.int32, h.tilem + 1)
-            out["tile_colidx"] = _arr(h.tile_colidx, np.int32, nt)
-            out["tile_rowidx"] = _arr(h.tile_rowidx, np.int32, nt)
-            out["tile_nnz"] = _arr(h.tile_nnz, np.int32, nt + 1)
-            out["csr_ptr"] = _arr(h.csr_ptr, np.uint8, nt * 16)
-            out["csr_col"] = _arr(h.csr_col, np.uint8, h.nnz)
-            out["val"] = _arr(h.val, np.float64, h.nnz)
-            out["mask"] = _arr(h.mask, np.uint16, nt * 16)
-            out["csc_ptr"] = _arr(h.csc_ptr, np.int32, h.tilen + 1) if h.csc_ptr else None
-            out["csc_rowidx"] = _arr(h.csc_rowidx, np.int32, nt) if h.csc_rowidx else None
-            out["csc_tileid"] = _arr(h.csc_tileid, np.int32, nt) if h.csc_tileid else None
-            return out
-        finally:
-            lib.tsg_htiles_free(C.byref(h))
+        return _take_htiles(h)
 
     # raw device int arrays
     def dev_ints(self, host: np.ndarray | None = None, n: int | None = None):
