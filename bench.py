@@ -146,6 +146,102 @@ def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotu
             "generate_s": round(gen_s, 2), "autotune": tuned}
 
 
+def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18, budget_gib=64):
+    """Serving workload: Poisson arrivals of A^2 / AA^T requests (50/50);
+    80% on Erdos-Renyi matrices (n log-uniform in [2k, 256k], mean nnz/row
+    U[4,32]), 20% on R-MAT (scale 18, edge factor 16), all registered once
+    (resident in HBM) under a 64 GiB budget.  Reports latency percentiles
+    (arrival -> result complete in HBM) at a fixed rate, the burst capacity
+    (all requests at once) and, on the same request list, one plain call
+    after another on the same resident operands."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    ers = []
+    for i in range(n_er):
+        n = int(round(np.exp(rng.uniform(np.log(2000), np.log(262144)))))
+        ers.append(ts.gen_er(n, float(rng.uniform(4, 32)), seed=1000 + i))
+    rms = [ts.gen_rmat(rmat_scale, 16, seed=2000 + i) for i in range(n_rmat)]
+    progress(f"serve: generated pool in {time.perf_counter() - t0:.1f} s")
+    mats = ers + rms
+    kinds = rng.uniform(size=n_req) < 0.8
+    pick = [int(rng.integers(0, n_er)) if k else n_er + int(rng.integers(0, n_rmat)) for k in kinds]
+    ops = [int(x) for x in rng.integers(0, 2, n_req)]
+    # sequential plain calls on resident operands (same kernels, same row index)
+    ctx = ts.Context(0)
+    dev = []
+    for a in mats:
+        da = ctx.upload(a)
+        ta = ctx.csr2tile(da)
+        ctx.build_csc(ta)
+        ctx.build_rowindex(ta)
+        dat = ctx.transpose(da)
+        tat = ctx.csr2tile(dat)
+        ctx.build_csc(tat)
+        ctx.build_rowindex(tat)
+        dev.append((da, ta, dat, tat))
+    plans = {}
+    for i, op in zip(pick, ops):
+        if (i, op) not in plans:
+            _, ta, _, tat = dev[i]
+            plans[(i, op)] = ctx.chunk_plan(ta, ta if op == ts.OP_A2 else tat, 1500000000)[0]
+    def one(i, op):
+        _, ta, _, tat = dev[i]
+        b = plans[(i, op)]
+        for r0, r1 in zip(b, b[1:]):
+            c, _ = ctx.spgemm_tile_rows(ta, ta if op == ts.OP_A2 else tat, r0, r1, timed=False)
+            ctx.free(c)
+        ctx.sync()
+    for key in plans:  # warm
+        one(*key)
+    t0 = time.perf_counter()
+    for i, op in zip(pick, ops):
+        one(i, op)
+    seq_s = time.perf_counter() - t0
+    progress(f"serve: sequential {seq_s:.2f} s for {n_req} requests")
+    for t in dev:
+        for o in t:
+            ctx.free(o)
+    ctx.trim()
+    ctx.close()
+    srv = ts.Server(0, ts.server_config(budget_bytes=budget_gib << 30))
+    t0 = time.perf_counter()
+    handles = [srv.register(a) for a in mats]
+    reg_s = time.perf_counter() - t0
+    progress(f"serve: registered {len(mats)} matrices in {reg_s:.1f} s")
+    hs = [handles[i] for i in pick]
+    # burst: every request at t = 0 (capacity)
+    t0 = time.perf_counter()
+    _, binfo = srv.replay(hs, ops, [0.0] * n_req, keep=False)
+    burst_s = time.perf_counter() - t0
+    progress(f"serve: burst {burst_s:.2f} s")
+    cap = n_req / burst_s
+    rate = lam if lam else 0.7 * cap
+    arr = np.cumsum(rng.exponential(1000.0 / rate, n_req)).tolist()
+    st0 = srv.stats()
+    t0 = time.perf_counter()
+    _, infos = srv.replay(hs, ops, arr, keep=False)
+    open_s = time.perf_counter() - t0
+    st1 = srv.stats()
+    srv.close()
+    lat = np.array([i.t_done_ms - i.t_arrival_ms for i in infos])
+    blat = np.array([i.t_done_ms - i.t_arrival_ms for i in binfo])
+    rm = np.array([not k for k in kinds])
+    return {"requests": n_req, "seed": seed, "pool": {"er": n_er, "rmat": n_rmat, "rmat_scale": rmat_scale},
+            "budget_gib": budget_gib, "register_s": round(reg_s, 2),
+            "rate_req_per_s": round(rate, 2), "solves_per_s": round(n_req / open_s, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "p99_ms_er": round(float(np.percentile(lat[~rm], 99)), 2) if (~rm).any() else None,
+            "p99_ms_rmat": round(float(np.percentile(lat[rm], 99)), 2) if rm.any() else None,
+            "burst_solves_per_s": round(cap, 2), "burst_p99_ms": round(float(np.percentile(blat, 99)), 2),
+            "sequential_solves_per_s": round(n_req / seq_s, 2),
+            "speedup_vs_sequential": round(seq_s / burst_s, 2),
+            "fused_launches": st1.fused_launches - st0.fused_launches,
+            "single_launches": st1.single_launches - st0.single_launches,
+            "peak_reserved_gib": round(st1.peak_reserved_bytes / 2 ** 30, 2),
+            "resident_gib": round(st1.resident_bytes / 2 ** 30, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +252,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-rmat-scale", type=int, default=18)
     ap.add_argument("--dense-threshold", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run the plain copy kernel over this many bytes (byte-counter reference)")
@@ -167,6 +267,10 @@ def main():
     import numpy as np
     import tilespgemm as ts
 
+    if args.serve_only:
+        print(json.dumps({"served": serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)}),
+              flush=True)
+        return
     if args.batch_only:
         ctx = ts.Context(0)
         print(json.dumps({"batched": batch_leg(ts, ctx, count=args.batch_count)}), flush=True)
@@ -242,6 +346,8 @@ def main():
         ctx.trim()
         out["batched"] = batch_leg(ts, ctx, count=args.batch_count)
     ctx.close()
+    if not args.no_serve:
+        out["served"] = serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)
     if not args.no_cpu:
         rp = a.rowptr.astype(np.int64)
         rowlen = np.diff(rp)

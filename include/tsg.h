@@ -120,10 +120,18 @@ typedef struct {
     int *csc_ptr;           /* tilen+1   (NULL until built) */
     int *csc_rowidx;        /* numtile */
     int *csc_tileid;        /* numtile */
+    int *rowx_ptr;          /* m+1   row index (NULL until built) */
+    int *rowx_col;          /* nnz: global columns, row by row ascending */
+    double *rowx_val;       /* nnz */
 } tsg_dtiles;
 
 int tsg_csr2tile(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dtiles *out);
 int tsg_tiles_build_csc(tsg_ctx *ctx, tsg_dtiles *t);
+/* Element row index of a B operand (its rows as CSR).  With it, the
+ * numeric step handles tile-rows too large or too sparse for the
+ * row-wise tile kernel element by element instead of by tile
+ * intersection.  Optional; 12 bytes per nonzero. */
+int tsg_tiles_build_rowindex(tsg_ctx *ctx, tsg_dtiles *t);
 int tsg_tile2csr(tsg_ctx *ctx, const tsg_dtiles *t, tsg_dcsr *out);
 void tsg_tiles_free(tsg_ctx *ctx, tsg_dtiles *t);
 
@@ -154,7 +162,11 @@ typedef struct {
                               row-wise LDS kernels (default 0 = by size) */
     int batch_group_nnz_cap; /* batches: most C nonzeros per fused group
                               (0 = the int32 limit of the tile offsets) */
-    int reserved[4];
+    int element_mode;      /* numeric step for tile-rows the row-wise tile
+                              kernel cannot hold, when B has a row index:
+                              0 = element-wise (default), 1 = tile-parallel,
+                              2 = element-wise for every tile-row */
+    int reserved[3];
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 
@@ -171,6 +183,21 @@ int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b,
                const tsg_tune *tune, tsg_dtiles *c, tsg_stats *stats);
 /* Step 1 alone: number of C tiles (structurally empty ones included). */
 int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, long long *out);
+
+/* Products whose C exceeds the int32 offsets of the tile format (or the
+ * memory at hand) run in chunks of A tile-rows.  The reference has no
+ * chunking: tilespgemm() computes the whole C at once
+ * (src/tilespgemm-cuda.h:2740-3124) with int offsets (src/common.h:23-25).
+ * tsg_spgemm_chunk_plan: tile-row bounds (bounds[0] = 0 < ... <
+ * bounds[*nchunks] = tilem, at most max_chunks chunks) so that each chunk
+ * has at most max_products intermediate products (an upper bound on its C
+ * nonzeros); products_total (may be NULL) = nnzCub.
+ * tsg_spgemm_tile_rows: C rows [16*r0, min(16*r1, m)) of A*B as a tile
+ * matrix of r1-r0 tile-rows, bit-identical to those rows of the full C. */
+int tsg_spgemm_chunk_plan(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, long long max_products,
+                          int max_chunks, int *bounds, int *nchunks, unsigned long long *products_total);
+int tsg_spgemm_tile_rows(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, int r0, int r1,
+                         const tsg_tune *tune, tsg_dtiles *c, tsg_stats *stats);
 
 /* ------------------------------------------------------------------ */
 /* Raw device memory helpers for host layers (all on the context stream;
@@ -270,6 +297,63 @@ typedef struct {
 int tsg_gen_batch_spec(int count, unsigned long long seed, int nmin, int nmax, tsg_batch_spec *out);
 /* Generates problem `s` (values U[-1,1)). */
 int tsg_gen_batch_problem(const tsg_batch_spec *s, tsg_hcsr *out);
+
+/* ------------------------------------------------------------------ */
+/* Serving: continuous batching of A^2 / AA^T requests on resident
+ * matrices.  There is no counterpart in the reference (its driver runs one
+ * product per process, src/main.cu:15-638).  Registration uploads a matrix
+ * once and keeps its CSR, tiles (with CSC and row indices) and those of
+ * A^T in HBM; it also runs each product once to size it exactly, and that
+ * symbolic size (C's tiles and nonzeros, plus the pipeline's work) is the
+ * request's admission estimate.  Worker threads (slots, one HIP stream
+ * each) take the first queued request that fits the budget left, fused
+ * with later small ones into one block-diagonal product; every result is
+ * bit-identical to tsg_spgemm on the same operands and stays in HBM until
+ * released.  All calls are thread-safe.                               */
+typedef struct tsg_server tsg_server;
+typedef struct {
+    long long budget_bytes;   /* HBM for resident operands + admitted requests (64 GiB) */
+    int slots;                /* worker streams (4) */
+    int max_batch;            /* requests fused into one product (64) */
+    long long small_bytes;    /* requests at most this large are fused (256 MiB) */
+    long long chunk_products; /* tile-row chunk cap on intermediate products (1.5e9) */
+    double starve_ms;         /* a head request waiting this long stops smaller
+                                 ones from passing it (200) */
+    tsg_tune tune;
+} tsg_server_config;
+void tsg_server_config_default(tsg_server_config *c);
+
+typedef struct {
+    int status, handle, op;
+    int chunks;               /* tile-row pieces of C */
+    int fused_with;           /* requests in the same fused product (1 = alone) */
+    long long numtileC, nnzC, bytes;
+    double t_arrival_ms, t_admit_ms, t_done_ms; /* server clock */
+} tsg_request_info;
+
+typedef struct {
+    long long resident_bytes, reserved_bytes, peak_reserved_bytes;
+    int queued;
+    long long fused_launches, single_launches;
+} tsg_server_stats_t;
+
+int tsg_server_create(int device, const tsg_server_config *cfg, tsg_server **out);
+void tsg_server_destroy(tsg_server *s);
+int tsg_server_register(tsg_server *s, const tsg_hcsr *a, int *handle);
+/* Exact size of a registered product (any pointer may be NULL). */
+int tsg_server_size(tsg_server *s, int handle, int op, long long *numtileC, long long *nnzC,
+                    long long *bytes, int *chunks);
+int tsg_server_submit(tsg_server *s, int handle, int op, unsigned long long *id);
+/* Blocks until request id is done (timeout_ms <= 0: no limit); returns its status. */
+int tsg_server_wait(tsg_server *s, unsigned long long id, double timeout_ms, tsg_request_info *info);
+int tsg_server_result_csr(tsg_server *s, unsigned long long id, tsg_hcsr *out);
+int tsg_server_release(tsg_server *s, unsigned long long id);
+int tsg_server_stats(tsg_server *s, tsg_server_stats_t *out);
+/* Open-loop load: request i submitted at arrival_ms[i] after the call;
+ * returns when all are done (infos may be NULL).  keep = 0 releases each
+ * result on completion. */
+int tsg_server_replay(tsg_server *s, int n, const int *handles, const int *ops, const double *arrival_ms,
+                      int keep, unsigned long long *ids, tsg_request_info *infos);
 
 /* Library build information (compile flags, target). */
 const char *tsg_build_info(void);

@@ -228,10 +228,12 @@ static int build_group(tsg_batch *b, const tsg_dcsr *probs, int first, int cnt, 
     st = tsg_csr2tile(ctx, &pa, &g.A);
     if (!st && all_a2) {
         st = tsg_tiles_build_csc(ctx, &g.A);
+        if (!st) st = tsg_tiles_build_rowindex(ctx, &g.A);
         g.B = g.A;
     } else if (!st) {
         st = tsg_csr2tile(ctx, &pb, &g.B);
         if (!st) st = tsg_tiles_build_csc(ctx, &g.B);
+        if (!st) st = tsg_tiles_build_rowindex(ctx, &g.B);
         tsg_dcsr_free(ctx, &pb);
     }
     tsg_dcsr_free(ctx, &pa);
@@ -393,16 +395,22 @@ extern "C" int tsg_batch_autotune(tsg_batch *b, int reps, tsg_tune *chosen, doub
     if (!b) { tsg_set_error("tsg_batch_autotune: null batch"); return TSG_ERR_ARG; }
     if (reps < 1) reps = 1;
     std::vector<tsg_tune> cands;
-    const int thresholds[] = {16, 64, 160, 257};
-    for (int fp = 0; fp < 2; fp++)
-        for (int th : thresholds) {
-            tsg_tune t;
-            tsg_tune_default(&t);
-            t.force_tile_parallel = fp;
-            t.dense_acc_min_nnz = th;
-            t.batch_group_nnz_cap = b->tune.batch_group_nnz_cap;
-            cands.push_back(t);
-        }
+    auto add = [&](int fp, int em, int th) {
+        tsg_tune t;
+        tsg_tune_default(&t);
+        t.force_tile_parallel = fp;
+        t.element_mode = em;
+        t.dense_acc_min_nnz = th;
+        t.batch_group_nnz_cap = b->tune.batch_group_nnz_cap;
+        cands.push_back(t);
+    };
+    // element-wise numeric step for the rows the row-wise kernel cannot
+    // hold, or for every row; the tile-parallel kernel with several
+    // dense-accumulator thresholds for those rows, or for every row
+    add(0, 0, 64);
+    add(0, 2, 64);
+    for (int th : {16, 64, 160, 257}) add(0, 1, th);
+    for (int th : {64, 257}) add(1, 1, th);
     double best = 1e30;
     tsg_tune bt = b->tune;
     for (const auto &t : cands) {
@@ -499,5 +507,193 @@ extern "C" int tsg_batch_result(tsg_batch *b, int i, tsg_htiles *tiles, tsg_hcsr
         tsg_dcsr_free(ctx, &d);
         TSG_TRY(st);
     }
+    return TSG_OK;
+}
+
+// ------------------------------------------------------------------ tile packing
+// Block-diagonal packing of resident tile matrices (no CSR round trip):
+// part p lands at tile-row TR_p, tile-column TC_p, tile TB_p and nonzero
+// NB_p of the packed matrix; every index array is shifted, every data
+// array copied.  Used by the server to fuse queued requests whose operands
+// are already in HBM as tiles.
+namespace {
+struct TilePart {
+    const int *tile_ptr, *tile_colidx, *tile_nnz, *csc_ptr, *csc_rowidx, *csc_tileid, *rowx_ptr, *rowx_col;
+    const unsigned char *csr_ptr, *csr_col;
+    const unsigned short *mask;
+    const double *val, *rowx_val;
+    int m, tilem, tilen, numtile, nnz;
+    int TR, TC, TB, NB;
+};
+
+__global__ void __launch_bounds__(256)
+k_tp_rows(const TilePart *__restrict__ P, const int *__restrict__ trb, int count, int tilem, int total_tiles,
+          int *__restrict__ tile_ptr) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I > tilem) return;
+    if (I == tilem) { tile_ptr[tilem] = total_tiles; return; }
+    const TilePart &p = P[find_block(trb, count, I)];
+    tile_ptr[I] = p.TB + p.tile_ptr[I - p.TR];
+}
+
+__global__ void __launch_bounds__(256)
+k_tp_cols(const TilePart *__restrict__ P, const int *__restrict__ tcb, int count, int tilen, int total_tiles,
+          int *__restrict__ csc_ptr) {
+    const int J = blockIdx.x * 256 + threadIdx.x;
+    if (J > tilen) return;
+    if (J == tilen) { csc_ptr[tilen] = total_tiles; return; }
+    const TilePart &p = P[find_block(tcb, count, J)];
+    const int lj = J - p.TC;
+    csc_ptr[J] = p.TB + (lj < p.tilen ? p.csc_ptr[lj] : p.numtile);
+}
+
+__global__ void __launch_bounds__(256)
+k_tp_tiles(const TilePart *__restrict__ P, const int *__restrict__ tbb, int count, int total_tiles,
+           int *__restrict__ colidx, int *__restrict__ rowidx, int *__restrict__ tnnz, int *__restrict__ cscrow,
+           int *__restrict__ cscid, uint4 *__restrict__ ptr16, uint4 *__restrict__ mask16) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= total_tiles) return;
+    const TilePart &p = P[find_block(tbb, count, t)];
+    const int lt = t - p.TB;
+    colidx[t] = p.tile_colidx[lt] + p.TC;
+    tnnz[t] = p.tile_nnz[lt] + p.NB;
+    if (rowidx) {
+        // tile row of local tile lt: last local tile-row whose start <= lt
+        int lo = 0, hi = p.tilem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.tile_ptr[mid] <= lt) lo = mid; else hi = mid - 1;
+        }
+        rowidx[t] = lo + p.TR;
+    }
+    if (cscrow) {
+        cscrow[t] = p.csc_rowidx[lt] + p.TR;
+        cscid[t] = p.csc_tileid[lt] + p.TB;
+    }
+    ptr16[t] = reinterpret_cast<const uint4 *>(p.csr_ptr)[lt];
+    mask16[2 * t] = reinterpret_cast<const uint4 *>(p.mask)[2 * lt];
+    mask16[2 * t + 1] = reinterpret_cast<const uint4 *>(p.mask)[2 * lt + 1];
+}
+
+__global__ void __launch_bounds__(256)
+k_tp_nnz(const TilePart *__restrict__ P, const int *__restrict__ nbb, int count, long long total_nnz,
+         unsigned char *__restrict__ col8, double *__restrict__ val, int *__restrict__ xcol, double *__restrict__ xval) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total_nnz) return;
+    const TilePart &p = P[find_block(nbb, count, (int)e)];
+    const int le = (int)(e - p.NB);
+    col8[e] = p.csr_col[le];
+    val[e] = p.val[le];
+    if (xcol) {
+        xcol[e] = p.rowx_col[le] + 16 * p.TC;
+        xval[e] = p.rowx_val[le];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_tp_xrows(const TilePart *__restrict__ P, const int *__restrict__ trb, int count, int m, int total_nnz,
+           int *__restrict__ xptr) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > m) return;
+    if (i == m) { xptr[m] = total_nnz; return; }
+    const TilePart &p = P[find_block(trb, count, i >> 4)];
+    const int li = i - 16 * p.TR;
+    xptr[i] = p.NB + (li < p.m ? p.rowx_ptr[li] : p.nnz);
+}
+}  // namespace
+
+// parts[i] packed at tile-row/tile-column bases; out gets the CSC index
+// and the row index when every part has them.  The packed matrix has
+// 16 * sum(tilem) rows and 16 * sum(tilen) columns.
+int tsg_tiles_pack(tsg_ctx *ctx, const tsg_dtiles *const *parts, int count, tsg_dtiles *out) {
+    memset(out, 0, sizeof(*out));
+    if (count <= 0) { tsg_set_error("tsg_tiles_pack: no parts"); return TSG_ERR_ARG; }
+    std::vector<TilePart> P(count);
+    std::vector<int> trb(count), tcb(count), tbb(count), nbb(count);
+    long long TR = 0, TC = 0, TB = 0, NB = 0;
+    bool csc = true, rowx = true;
+    for (int i = 0; i < count; i++) {
+        const tsg_dtiles *t = parts[i];
+        csc = csc && t->csc_ptr;
+        rowx = rowx && t->rowx_ptr;
+        P[i] = TilePart{t->tile_ptr, t->tile_colidx, t->tile_nnz, t->csc_ptr, t->csc_rowidx, t->csc_tileid,
+                        t->rowx_ptr, t->rowx_col, t->csr_ptr, t->csr_col, t->mask, t->val, t->rowx_val,
+                        t->m, t->tilem, t->tilen, t->numtile, t->nnz,
+                        (int)TR, (int)TC, (int)TB, (int)NB};
+        trb[i] = (int)TR; tcb[i] = (int)TC; tbb[i] = (int)TB; nbb[i] = (int)NB;
+        TR += t->tilem; TC += t->tilen; TB += t->numtile; NB += t->nnz;
+        if (16 * TR > 0x7fffffffLL || 16 * TC > 0x7fffffffLL || TB > 0x7fffffffLL || NB > 0x7fffffffLL) {
+            tsg_set_error("tsg_tiles_pack: packed matrix exceeds int32 indices");
+            return TSG_ERR_OVERFLOW;
+        }
+    }
+    out->tilem = (int)TR; out->tilen = (int)TC; out->m = (int)(16 * TR); out->n = (int)(16 * TC);
+    out->numtile = (int)TB; out->nnz = (int)NB;
+    TilePart *dP;
+    int *dtrb, *dtcb, *dtbb, *dnbb;
+    hipStream_t s = ctx->stream;
+    TSG_TRY(tsg_alloc_t(ctx, &dP, (size_t)count));
+    TSG_TRY(tsg_alloc_t(ctx, &dtrb, (size_t)count));
+    TSG_TRY(tsg_alloc_t(ctx, &dtcb, (size_t)count));
+    TSG_TRY(tsg_alloc_t(ctx, &dtbb, (size_t)count));
+    TSG_TRY(tsg_alloc_t(ctx, &dnbb, (size_t)count));
+    TSG_HIP(hipMemcpyAsync(dP, P.data(), sizeof(TilePart) * count, hipMemcpyHostToDevice, s));
+    TSG_HIP(hipMemcpyAsync(dtrb, trb.data(), 4 * (size_t)count, hipMemcpyHostToDevice, s));
+    TSG_HIP(hipMemcpyAsync(dtcb, tcb.data(), 4 * (size_t)count, hipMemcpyHostToDevice, s));
+    TSG_HIP(hipMemcpyAsync(dtbb, tbb.data(), 4 * (size_t)count, hipMemcpyHostToDevice, s));
+    TSG_HIP(hipMemcpyAsync(dnbb, nbb.data(), 4 * (size_t)count, hipMemcpyHostToDevice, s));
+    TSG_TRY(tsg_alloc_t(ctx, &out->tile_ptr, (size_t)TR + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &out->tile_colidx, (size_t)TB + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &out->tile_rowidx, (size_t)TB + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &out->tile_nnz, (size_t)TB + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &out->csr_ptr, (size_t)TB * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &out->mask, (size_t)TB * 16 + 16));
+    TSG_TRY(tsg_alloc_t(ctx, &out->csr_col, (size_t)NB + 1));
+    TSG_TRY(tsg_alloc_t(ctx, &out->val, (size_t)NB + 1));
+    if (csc) {
+        TSG_TRY(tsg_alloc_t(ctx, &out->csc_ptr, (size_t)TC + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &out->csc_rowidx, (size_t)TB + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &out->csc_tileid, (size_t)TB + 1));
+    }
+    if (rowx) {
+        TSG_TRY(tsg_alloc_t(ctx, &out->rowx_ptr, (size_t)out->m + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &out->rowx_col, (size_t)NB + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &out->rowx_val, (size_t)NB + 1));
+    }
+    hipLaunchKernelGGL(k_tp_rows, dim3(tsg_div_up(TR + 1, 256)), dim3(256), 0, s, (const TilePart *)dP,
+                       (const int *)dtrb, count, (int)TR, (int)TB, out->tile_ptr);
+    TSG_LAUNCH_CHECK();
+    // the tile-row of every tile (k_tp_tiles searches each part's tile_ptr)
+    if (TB > 0) {
+        hipLaunchKernelGGL(k_tp_tiles, dim3(tsg_div_up(TB, 256)), dim3(256), 0, s, (const TilePart *)dP,
+                           (const int *)dtbb, count, (int)TB, out->tile_colidx, out->tile_rowidx, out->tile_nnz,
+                           csc ? out->csc_rowidx : (int *)nullptr, csc ? out->csc_tileid : (int *)nullptr,
+                           reinterpret_cast<uint4 *>(out->csr_ptr), reinterpret_cast<uint4 *>(out->mask));
+        TSG_LAUNCH_CHECK();
+    }
+    // tile_nnz[TB] = NB
+    TSG_HIP(hipMemcpyAsync(out->tile_nnz + TB, &out->nnz, 4, hipMemcpyHostToDevice, s));
+    if (csc) {
+        hipLaunchKernelGGL(k_tp_cols, dim3(tsg_div_up(TC + 1, 256)), dim3(256), 0, s, (const TilePart *)dP,
+                           (const int *)dtcb, count, (int)TC, (int)TB, out->csc_ptr);
+        TSG_LAUNCH_CHECK();
+    }
+    if (NB > 0) {
+        hipLaunchKernelGGL(k_tp_nnz, dim3(tsg_div_up(NB, 256)), dim3(256), 0, s, (const TilePart *)dP,
+                           (const int *)dnbb, count, NB, out->csr_col, out->val,
+                           rowx ? out->rowx_col : (int *)nullptr, rowx ? out->rowx_val : (double *)nullptr);
+        TSG_LAUNCH_CHECK();
+    }
+    if (rowx) {
+        hipLaunchKernelGGL(k_tp_xrows, dim3(tsg_div_up((long long)out->m + 1, 256)), dim3(256), 0, s,
+                           (const TilePart *)dP, (const int *)dtrb, count, out->m, (int)NB, out->rowx_ptr);
+        TSG_LAUNCH_CHECK();
+    }
+    TSG_HIP(hipStreamSynchronize(s));  // host descriptors and out->nnz go out of scope
+    tsg_free(ctx, dP);
+    tsg_free(ctx, dtrb);
+    tsg_free(ctx, dtcb);
+    tsg_free(ctx, dtbb);
+    tsg_free(ctx, dnbb);
     return TSG_OK;
 }

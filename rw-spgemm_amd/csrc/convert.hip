@@ -376,10 +376,21 @@ int tsg_tile2csr_view(tsg_ctx *ctx, const tsg_dtiles *t, int jbase, tsg_dcsr *ou
     return TSG_OK;
 }
 
+extern "C" int tsg_tiles_build_rowindex(tsg_ctx *ctx, tsg_dtiles *t) {
+    if (t->rowx_ptr) return TSG_OK;
+    tsg_dcsr d;
+    TSG_TRY(tsg_tile2csr_view(ctx, t, 0, &d));
+    t->rowx_ptr = d.rowptr;
+    t->rowx_col = d.colidx;
+    t->rowx_val = d.val;
+    return TSG_OK;
+}
+
 extern "C" void tsg_tiles_free(tsg_ctx *ctx, tsg_dtiles *t) {
     if (!t) return;
     void *ps[] = {t->tile_ptr, t->tile_colidx, t->tile_rowidx, t->tile_nnz, t->csr_ptr, t->csr_col,
-                  t->val, t->mask, t->csc_ptr, t->csc_rowidx, t->csc_tileid};
+                  t->val, t->mask, t->csc_ptr, t->csc_rowidx, t->csc_tileid, t->rowx_ptr, t->rowx_col,
+                  t->rowx_val};
     for (void *p : ps) tsg_free(ctx, p);
     memset(t, 0, sizeof(*t));
 }
@@ -583,6 +594,8 @@ extern "C" int tsg_tiles_csc_order(tsg_ctx *ctx, tsg_dtiles *t, tsg_dtiles *o) {
     o->val = nullptr;
     o->mask = nullptr;
     o->csc_ptr = o->csc_rowidx = o->csc_tileid = nullptr;
+    o->rowx_ptr = o->rowx_col = nullptr;
+    o->rowx_val = nullptr;
     TSG_TRY(tsg_alloc_t(ctx, &o->tile_ptr, (size_t)t->tilem + 1));
     TSG_TRY(tsg_alloc_t(ctx, &o->tile_colidx, (size_t)nt + 1));
     TSG_TRY(tsg_alloc_t(ctx, &o->tile_rowidx, (size_t)nt + 1));

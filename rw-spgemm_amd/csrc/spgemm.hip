@@ -196,20 +196,91 @@ __device__ __forceinline__ void for_each_pair(const TileOperands &op, int I, int
     if (staged) tsg_wave_sync();
 }
 
+// ------------------------------------------------------------------ segments
+// A segment is a range [s0, s1) of the C tiles of one tile-row I, stored
+// as int4 {I, s0, s1, 0}.  Long tile-rows are cut into segments so each
+// fits a block's LDS; a segment's C tiles cover the tile columns
+// [Ccol[c0+s0], Ccol[c0+s1-1]], and only the B tiles inside that column
+// range meet them.  MODE 0 cuts by tile count; MODE 1 also keeps every
+// segment within max_nnz C nonzeros (tile nnz <= 256 <= max_nnz).  Only
+// tile-rows with rowcls == want (want < 0: all) are cut.
+template <int MODE>
+__device__ __forceinline__ int seg_walk(const int *__restrict__ Cnnz, int c0, int u, int max_tiles, int max_nnz,
+                                        int4 *out, int I) {
+    if (MODE == 0) {
+        const int n = (u + max_tiles - 1) / max_tiles;
+        if (out)
+            for (int j = 0; j < n; j++) out[j] = make_int4(I, j * max_tiles, min(u, (j + 1) * max_tiles), 0);
+        return n;
+    }
+    int n = 0, s0 = 0, acc = 0;
+    for (int s = 0; s < u; s++) {
+        const int tn = Cnnz[c0 + s + 1] - Cnnz[c0 + s];
+        if (s > s0 && (s - s0 == max_tiles || acc + tn > max_nnz)) {
+            if (out) out[n] = make_int4(I, s0, s, 0);
+            n++;
+            s0 = s;
+            acc = 0;
+        }
+        acc += tn;
+    }
+    if (u > 0) {
+        if (out) out[n] = make_int4(I, s0, u, 0);
+        n++;
+    }
+    return n;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_seg(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem, const unsigned char *__restrict__ rowcls,
+      int want, int max_tiles, int max_nnz, int *__restrict__ cnt_or_off, int4 *__restrict__ items) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= tilem) return;
+    const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
+    const bool take = u > 0 && (want < 0 || rowcls[I] == want);
+    if (!items) {
+        cnt_or_off[I] = take ? seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, nullptr, I) : 0;
+    } else if (take) {
+        seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, items + cnt_or_off[I], I);
+    }
+}
+
+// Upper bound on the segments of a C with tilem tile-rows and numtile
+// tiles: one per tile-row, one per max_tiles tiles closed by count, and
+// (MODE 1) one per (max_nnz - 255) nonzeros closed by size, with at most
+// 256 nonzeros per tile.
+static long long seg_bound(long long tilem, long long numtile, int max_tiles, int max_nnz, int mode) {
+    long long b = tilem + numtile / max_tiles + 1;
+    if (mode == 1) b += 256 * numtile / (max_nnz - 255) + 1;
+    return b;
+}
+
+// First index in [lo, hi) with list[index] >= key (list ascending).
+__device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int lo, int hi, int key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------ step 2
-// Row-wise (Gustavson) bitmask symbolic: one 256-thread block per C
-// tile-row.  Every (A tile (I,K), B tile (K,J)) pair of the row -- the same
-// pairs step 1 enumerated -- is visited exactly once, flattened over the
-// threads; no intersection search is needed.  Row r of the pair's product
-// pattern, OR over k in Amask[r] of Bmask[k], is OR-ed into C tile J's row
-// mask held in LDS (OR is order-free, so LDS atomics are exact).
+// Row-wise (Gustavson) bitmask symbolic: one 256-thread block per segment
+// of a C tile-row.  Every (A tile (I,K), B tile (K,J)) pair of the row
+// with J in the segment's column range -- the pairs step 1 enumerated --
+// is visited exactly once, flattened over the threads; no intersection
+// search is needed.  Row r of the pair's product pattern, OR over k in
+// Amask[r] of Bmask[k], is OR-ed into C tile J's row mask held in LDS (OR
+// is order-free, so LDS atomics are exact).
 #define G2_THREADS 256
-#define G2_UMAX 512    // C tiles of one tile-row held in LDS
+#define G2_UMAX 512    // C tiles of one segment held in LDS
 #define G2_PCHUNK 256  // A tiles per enumeration round
 
 __global__ void __launch_bounds__(G2_THREADS)
-k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
-             unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz) {
+k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restrict__ nseg, const int *__restrict__ Cptr,
+             const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
+             int *__restrict__ Cnnz) {
     __shared__ int s_ccol[G2_UMAX];
     __shared__ unsigned s_m[G2_UMAX * 8];  // two 16-bit rows per word, even row high
     __shared__ int s_pre[G2_PCHUNK];
@@ -217,12 +288,15 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     __shared__ int s_tot;
     __shared__ unsigned short s_bst[G2_THREADS][16];
     const int tid = threadIdx.x;
-    const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
-    if (I >= tilem) return;
-    const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
-    if (u == 0 || u > G2_UMAX) return;  // block-uniform
+    const int w = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
+    if (w >= *nseg) return;  // block-uniform
+    const int4 sg = segs[w];
+    const int I = sg.x;
+    const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
+    const bool whole = sg.y == 0 && sg.z == Cptr[I + 1] - Cptr[I];
     for (int i = tid; i < u; i += G2_THREADS) s_ccol[i] = Ccol[c0 + i];
     for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
+    const int jlo = Ccol[c0], jhi = Ccol[c0 + u - 1];
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
     unsigned short *bst = s_bst[tid];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
@@ -231,7 +305,12 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (tid < np) {
             const int K = op.Acol[pc + tid];
             b0 = op.Bptr[K];
-            len = op.Bptr[K + 1] - b0;
+            int b1 = op.Bptr[K + 1];
+            if (!whole) {  // B tiles of row K inside the segment's columns
+                b0 = lower_bound_i(op.Bcol, b0, b1, jlo);
+                b1 = lower_bound_i(op.Bcol, b0, b1, jhi + 1);
+            }
+            len = b1 - b0;
         }
         // block-wide exclusive scan of len
         __shared__ int s_wsum[G2_THREADS / 64];
@@ -239,7 +318,7 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
         __syncthreads();
         int woff = 0;
-        for (int w = 0; w < (tid >> 6); w++) woff += s_wsum[w];
+        for (int w2 = 0; w2 < (tid >> 6); w2++) woff += s_wsum[w2];
         s_pre[tid] = woff + incl - len;
         s_b0[tid] = b0;
         if (tid == G2_THREADS - 1) s_tot = woff + incl;
@@ -252,7 +331,7 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
                 if (p + st < G2_PCHUNK && s_pre[p + st] <= x) p += st;
             const int q = s_b0[p] + (x - s_pre[p]);
             const int J = op.Bcol[q];
-            int lo = 0, hi = u - 1;  // J is in the step-1 list
+            int lo = 0, hi = u - 1;  // J is in the segment's step-1 list
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
@@ -282,15 +361,15 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     }
     // per C tile: popcounts, u8 row offsets, tile nnz
     for (int s = tid; s < u; s += G2_THREADS) {
-        unsigned w[8];
+        unsigned w8[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) w[k] = s_m[s * 8 + k];
+        for (int k = 0; k < 8; k++) w8[k] = s_m[s * 8 + k];
         unsigned mk[8], pt[4];
         int sum = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const unsigned hi = w[k] >> 16, lo = w[k] & 0xffffu;  // rows 2k, 2k+1
-            mk[k] = hi | (lo << 16);                             // little-endian u16 pair
+            const unsigned hi = w8[k] >> 16, lo = w8[k] & 0xffffu;  // rows 2k, 2k+1
+            mk[k] = hi | (lo << 16);                               // little-endian u16 pair
         }
 #pragma unroll
         for (int q4 = 0; q4 < 4; q4++) {
@@ -298,7 +377,7 @@ k_step2_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int r = q4 * 4 + b;
-                const unsigned row = (r & 1) ? (w[r >> 1] & 0xffffu) : (w[r >> 1] >> 16);
+                const unsigned row = (r & 1) ? (w8[r >> 1] & 0xffffu) : (w8[r >> 1] >> 16);
                 o |= (unsigned)sum << (8 * b);
                 sum += __popc(row);
             }
@@ -470,6 +549,98 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
         const int c = 15 - bb;
         outv[j] = dense ? accr[c] : accg[cp + j];
         outc[j] = (unsigned char)c;
+    }
+}
+
+// ------------------------------------------------------------------ step 3, element-wise
+// Segments of tile-rows the row-wise tile kernel cannot hold (too many C
+// tiles, A tiles or B tiles, typically hypersparse rows where most tiles
+// carry one or two nonzeros): element-level Gustavson with B's row index.
+// One 256-thread block per segment (<= G3E_UMAX C tiles, <= G3E_NNZ C
+// nonzeros, accumulated in LDS in final order); 16 groups of 16 lanes,
+// group r owns C row 16I+r.  The group walks A row 16I+r in ascending k
+// (A tiles in ascending K, bits in ascending column) and, for each a_ik,
+// its lanes scatter the entries of B row k that fall in the segment's
+// column range: distinct C entries per step, steps in ascending k, so each
+// C entry again sums its products in the serial SPA order, without atomics.
+#define G3E_UMAX 256
+#define G3E_NNZ 2048
+
+__global__ void __launch_bounds__(256)
+k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
+             const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
+             const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
+             const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
+             double *__restrict__ Cval) {
+    __shared__ int s_ccol[G3E_UMAX];
+    __shared__ int s_coff[G3E_UMAX];
+    __shared__ unsigned short s_cmask[G3E_UMAX * 16];
+    __shared__ unsigned char s_cptr[G3E_UMAX * 16];
+    __shared__ double s_acc[G3E_NNZ];
+    const int tid = threadIdx.x;
+    const int w = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
+    if (w >= *nseg) return;  // block-uniform
+    const int4 sg = segs[w];
+    const int I = sg.x;
+    const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
+    const int nbase = Cnnz[c0], nseg_nnz = Cnnz[c0 + u] - nbase;
+    for (int i = tid; i < nseg_nnz; i += 256) s_acc[i] = 0.0;
+    for (int i = tid; i < u; i += 256) {
+        s_ccol[i] = Ccol[c0 + i];
+        s_coff[i] = Cnnz[c0 + i] - nbase;
+        const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
+        reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
+        reinterpret_cast<uint4 *>(s_cmask)[2 * i + 1] = cm[1];
+        reinterpret_cast<uint4 *>(s_cptr)[i] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + i) * 16)[0];
+    }
+    __syncthreads();
+    const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
+    const int r = tid >> 4, l = tid & 15;
+    const int row = 16 * I + r;
+    if (row < m && nseg_nnz > 0) {
+        const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
+        for (int t = a0; t < a1; t++) {
+            unsigned am = op.Amask[(long long)t * 16 + r];
+            if (!am) continue;
+            const double *av = op.Aval + op.Annz[t] + op.Aptr16[(long long)t * 16 + r];
+            const int k16 = op.Acol[t] * 16;
+            int ia = 0;
+            while (am) {
+                const int b = 31 - __clz(am);
+                am &= ~(1u << b);
+                const int k = k16 + 15 - b;
+                const double a = av[ia++];
+                const int e1 = Browp[k + 1];
+                const int e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
+                for (int e = e0 + l; e < e1; e += 16) {
+                    const int col = Bcolx[e];
+                    if (col >= col_hi) break;
+                    const int J = col >> 4, c = col & 15;
+                    int lo = 0, hi = u - 1;  // J is in the segment's tile list
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+                    }
+                    const unsigned cm = s_cmask[lo * 16 + r];
+                    double *acc = &s_acc[s_coff[lo] + s_cptr[lo * 16 + r] + __popc(cm >> (16 - c))];
+                    *acc = __builtin_fma(a, Bvalx[e], *acc);
+                }
+                tsg_wave_sync();  // the next k of this row reads what lanes wrote
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < nseg_nnz; i += 256) Cval[(long long)nbase + i] = s_acc[i];
+    for (int x = tid; x < u * 16; x += 256) {
+        const int slot = x >> 4;
+        unsigned cm = s_cmask[x];
+        unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
+        int jj = 0;
+        while (cm) {
+            const int bb = 31 - __clz(cm);
+            cm &= ~(1u << bb);
+            oc[jj++] = (unsigned char)(15 - bb);
+        }
     }
 }
 
@@ -787,7 +958,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 // tile-row, B nonzeros per B tile-row, A tile nonzeros) for the LDS layout.
 __global__ void __launch_bounds__(256)
 k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
-            unsigned char *__restrict__ cls, int *__restrict__ out) {
+            unsigned char *__restrict__ cls, int *__restrict__ out, int none_fit) {
     const int I = blockIdx.x * 256 + threadIdx.x;
     int unfit = 0, mu = 0, mn = 0, ml = 0, mq = 0, mb = 0, ma = 0;
     if (I < tilem) {
@@ -798,7 +969,7 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
             k = 2;
         } else {
             const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-            bool ok = c1 - c0 <= G3_UMAX && nrow <= G3_NNZMAX && a1 - a0 <= G3_LAMAX;
+            bool ok = !none_fit && c1 - c0 <= G3_UMAX && nrow <= G3_NNZMAX && a1 - a0 <= G3_LAMAX;
             int lq = 0, lbn = 0, lan = 0;
             for (int p = a0; ok && p < a1; p++) {
                 const int K = op.Acol[p];
@@ -881,50 +1052,126 @@ static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B,
     return TSG_OK;
 }
 
-static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P) {
-    if (C->numtile <= 0) return TSG_OK;
-    if (!P.all) {
-        hipLaunchKernelGGL(k_step2_rows, dim3(C->tilem), dim3(G2_THREADS), 0, s, op, C->tile_ptr, C->tile_colidx,
-                           C->tilem, C->mask, C->csr_ptr, C->tile_nnz);
-        TSG_LAUNCH_CHECK();
+// Device scratch of one SpGEMM beyond C itself.
+struct SpgemmWork {
+    long long *part1;  // scan partials: C tile pointers
+    long long *part2;  // scan partials: C tile nnz
+    long long *part3;  // scan partials: segment counts
+    int *segcnt;       // tilem+1 segment counts -> offsets (total at [tilem])
+    unsigned char *rowcls;
+    int *cls_out;
+    int4 *seg2;        // step-2 segments
+    int4 *seg3;        // step-3 element segments
+    long long seg2_cap, seg3_cap;
+};
+
+static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Offsets of the work arrays inside one buffer; returns its size.  The
+// step-3 segments go last (the synchronous pipeline adds them only once
+// C's nonzero count is known).
+static size_t work_layout(long long tilem, long long numtile, long long nnz, size_t off[8]) {
+    const size_t sz[8] = {
+        8 * (size_t)tsg_scan_partial_count(tilem), 8 * (size_t)tsg_scan_partial_count(numtile),
+        8 * (size_t)tsg_scan_partial_count(tilem), 4 * ((size_t)tilem + 1), (size_t)tilem + 1, 32,
+        16 * (size_t)seg_bound(tilem, numtile, G2_UMAX, 0, 0),
+        16 * (size_t)(tilem + numtile / G3E_UMAX + nnz / (G3E_NNZ - 255) + 2)};
+    size_t o = 0;
+    for (int i = 0; i < 8; i++) {
+        off[i] = o;
+        o += up16(sz[i]);
     }
-    if (P.all || P.maxrow > G2_UMAX) {
-        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
-        hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
-                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, P.all);
-        TSG_LAUNCH_CHECK();
-    }
-    return TSG_OK;
+    return o;
 }
 
-static int launch_row_class(hipStream_t s, const TileOperands &op, tsg_dtiles *C, unsigned char *rowcls,
-                            int *cls_out) {
-    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
-                       C->tilem, rowcls, cls_out);
+static SpgemmWork work_at(void *base, long long tilem, long long numtile, long long nnz, void *seg3_base) {
+    size_t off[8];
+    work_layout(tilem, numtile, nnz, off);
+    char *b = (char *)base;
+    SpgemmWork w;
+    w.part1 = (long long *)(b + off[0]);
+    w.part2 = (long long *)(b + off[1]);
+    w.part3 = (long long *)(b + off[2]);
+    w.segcnt = (int *)(b + off[3]);
+    w.rowcls = (unsigned char *)(b + off[4]);
+    w.cls_out = (int *)(b + off[5]);
+    w.seg2 = (int4 *)(b + off[6]);
+    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[7]);
+    w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
+    w.seg3_cap = tilem + numtile / G3E_UMAX + nnz / (G3E_NNZ - 255) + 2;
+    return w;
+}
+
+// Segments of the selected tile-rows into items; the count lands in
+// w.segcnt[tilem].
+template <int MODE>
+static int launch_segments(hipStream_t s, const tsg_dtiles *C, const SpgemmWork &w, int want, int max_tiles,
+                           int max_nnz, int4 *items) {
+    const unsigned g = tsg_div_up(C->tilem, 256);
+    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem,
+                       (const unsigned char *)w.rowcls, want, max_tiles, max_nnz, w.segcnt, (int4 *)nullptr);
+    TSG_LAUNCH_CHECK();
+    TSG_TRY(tsg_scan_i32_enqueue(s, w.segcnt, w.segcnt, C->tilem, w.part3));
+    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem,
+                       (const unsigned char *)w.rowcls, want, max_tiles, max_nnz, w.segcnt, items);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
 
-static int launch_step3(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P,
-                        const unsigned char *rowcls, hipEvent_t *ev = nullptr) {
+static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P,
+                        const SpgemmWork &w) {
+    if (C->numtile <= 0) return TSG_OK;
+    if (P.all) {
+        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
+        hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
+                           C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, 1);
+        TSG_LAUNCH_CHECK();
+        return TSG_OK;
+    }
+    TSG_TRY(launch_segments<0>(s, C, w, -1, G2_UMAX, 0, w.seg2));
+    hipLaunchKernelGGL(k_step2_rows, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), 0, s, op, (const int4 *)w.seg2,
+                       (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
+                       C->tile_nnz);
+    TSG_LAUNCH_CHECK();
+    return TSG_OK;
+}
+
+static int launch_row_class(hipStream_t s, const TileOperands &op, tsg_dtiles *C, unsigned char *rowcls,
+                            int *cls_out, int none_fit) {
+    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
+                       C->tilem, rowcls, cls_out, none_fit);
+    TSG_LAUNCH_CHECK();
+    return TSG_OK;
+}
+
+static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles *B, tsg_dtiles *C,
+                        const tsg_spgemm_plan &P, const SpgemmWork &w, hipEvent_t *ev = nullptr) {
     if (C->nnz <= 0) return TSG_OK;
     if (!P.all && P.s3_rows) {
-        if (ev) TSG_HIP(hipEventRecord(ev[0], s));
         G3Layout L;
         L.umax = P.g3[0]; L.nnzmax = P.g3[1]; L.lamax = P.g3[2];
         L.qb = P.g3[3]; L.bnnz = P.g3[4]; L.amax = P.g3[5];
         size_t off[G3_NARR];
         const size_t lds = g3_layout(L, off);
+        if (ev) TSG_HIP(hipEventRecord(ev[0], s));
         hipLaunchKernelGGL(k_step3_rows, dim3(C->tilem), dim3(G3_THREADS), lds, s, op, C->tile_ptr, C->tile_colidx,
-                           C->tilem, C->mask, C->csr_ptr, C->tile_nnz, rowcls, C->csr_col, C->val, L);
+                           C->tilem, C->mask, C->csr_ptr, C->tile_nnz, (const unsigned char *)w.rowcls, C->csr_col,
+                           C->val, L);
         TSG_LAUNCH_CHECK();
         if (ev) TSG_HIP(hipEventRecord(ev[1], s));
     }
-    if (P.all || P.s3_unfit) {
+    if (!P.all && P.s3_unfit && P.s3_elem) {
+        TSG_TRY(launch_segments<1>(s, C, w, 0, G3E_UMAX, G3E_NNZ, w.seg3));
+        hipLaunchKernelGGL(k_step3_elem, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
+                           (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
+                           (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
+                           C->tile_nnz, C->csr_col, C->val);
+        TSG_LAUNCH_CHECK();
+    } else if (P.all || P.s3_unfit) {
         const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
         hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
                            C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val,
-                           P.dense_min, P.all, rowcls);
+                           P.dense_min, P.all, (const unsigned char *)w.rowcls);
         TSG_LAUNCH_CHECK();
     }
     return TSG_OK;
@@ -954,6 +1201,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     memset(&P, 0, sizeof(P));
     P.all = tune.force_tile_parallel ? 1 : 0;
     P.dense_min = tune.dense_acc_min_nnz;
+    P.s3_elem = B->rowx_ptr != nullptr && tune.element_mode != 1;
+    P.elem_all = B->rowx_ptr != nullptr && tune.element_mode == 2;
     hipStream_t s = ctx->stream;
 
     hipEvent_t ev[6];
@@ -961,9 +1210,9 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     if (timed) {
         memset(stats, 0, sizeof(*stats));
         for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
+        TSG_HIP(hipEventRecord(ev[0], s));
         TSG_HIP(hipEventRecord(ev[4], s));
         TSG_HIP(hipEventRecord(ev[5], s));
-        TSG_HIP(hipEventRecord(ev[0], s));
     }
     C->m = A->m;
     C->n = B->n;
@@ -975,26 +1224,23 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     void *scr;
     TSG_TRY(tsg_alloc(ctx, &scr, 64));
     unsigned long long *bins = (unsigned long long *)scr;
-    int *maxrow = (int *)(bins + 4);
     TSG_HIP(hipMemsetAsync(scr, 0, 64, s));
     long long numtileC = 0;
     TSG_TRY(tsg_scan_i32(ctx, C->tile_ptr, C->tile_ptr, C->tilem, &numtileC));
     P.numtileC = numtileC;
     C->numtile = (int)numtileC;
     TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
+    void *wbuf = nullptr, *w3buf = nullptr;
+    size_t woff[8];
+    const size_t wbytes = work_layout(C->tilem, numtileC, 0, woff);
+    TSG_TRY(tsg_alloc(ctx, &wbuf, woff[7]));  // everything but the step-3 segments
+    (void)wbytes;
+    SpgemmWork w = work_at(wbuf, C->tilem, numtileC, 0, nullptr);
     TSG_TRY(launch_step1(s, A, B, C, true));
-    if (numtileC > 0) {
-        hipLaunchKernelGGL(k_max_rowlen, dim3(std::min<unsigned>(1024, tsg_div_up(C->tilem, 256))), dim3(256), 0, s,
-                           C->tile_ptr, C->tilem, maxrow);
-        TSG_LAUNCH_CHECK();
-        long long mx = 0;
-        TSG_TRY(tsg_read_scalar_i64(ctx, maxrow, 0, &mx));
-        P.maxrow = (int)mx;
-    }
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
     // ---- step 2
     const TileOperands op = make_operands(A, B);
-    TSG_TRY(launch_step2(s, op, C, P));
+    TSG_TRY(launch_step2(s, op, C, P, w));
     if (timed && numtileC > 0) {
         hipLaunchKernelGGL(k_bins, dim3(std::min<unsigned>(2048, tsg_div_up(numtileC, 256))), dim3(256), 0, s,
                            C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
@@ -1009,23 +1255,23 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
     TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
     if (nnzC > 0) {
-        unsigned char *rowcls = nullptr;
-        int *cls_out = nullptr;
-        TSG_TRY(tsg_alloc_t(ctx, &rowcls, (size_t)C->tilem + 1));
-        TSG_TRY(tsg_alloc_t(ctx, &cls_out, 8));
         if (!P.all) {
-            TSG_HIP(hipMemsetAsync(cls_out, 0, 8 * sizeof(int), s));
-            TSG_TRY(launch_row_class(s, op, C, rowcls, cls_out));
+            TSG_HIP(hipMemsetAsync(w.cls_out, 0, 8 * sizeof(int), s));
+            TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
             int h[8];
-            TSG_HIP(hipMemcpyAsync(h, cls_out, sizeof(h), hipMemcpyDeviceToHost, s));
+            TSG_HIP(hipMemcpyAsync(h, w.cls_out, sizeof(h), hipMemcpyDeviceToHost, s));
             TSG_HIP(hipStreamSynchronize(s));
             P.s3_unfit = h[0] > 0;
             P.s3_rows = h[1] > 0;
             for (int i = 0; i < 6; i++) P.g3[i] = std::max(1, h[1 + i]);
         }
-        TSG_TRY(launch_step3(s, op, C, P, rowcls, timed ? ev + 4 : nullptr));
-        tsg_free(ctx, rowcls);
-        tsg_free(ctx, cls_out);
+        if (!P.all && P.s3_unfit && P.s3_elem) {
+            const SpgemmWork w3 = work_at(wbuf, C->tilem, numtileC, nnzC, nullptr);
+            TSG_TRY(tsg_alloc(ctx, &w3buf, 16 * (size_t)w3.seg3_cap));
+            w.seg3 = (int4 *)w3buf;
+            w.seg3_cap = w3.seg3_cap;
+        }
+        TSG_TRY(launch_step3(s, op, B, C, P, w, timed ? ev + 4 : nullptr));
     }
     if (timed) {
         TSG_HIP(hipEventRecord(ev[3], s));
@@ -1045,6 +1291,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         stats->nnzC = nnzC;
         for (auto &e : ev) hipEventDestroy(e);
     }
+    tsg_free(ctx, w3buf);
+    tsg_free(ctx, wbuf);
     tsg_free(ctx, scr);
     if (plan_out) *plan_out = P;
     return TSG_OK;
@@ -1056,25 +1304,136 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
 }
 
 size_t tsg_spgemm_work_bytes(const tsg_dtiles *C) {
-    return ((size_t)tsg_scan_partial_count(C->tilem) + (size_t)tsg_scan_partial_count(C->numtile)) * 8 +
-           (((size_t)C->tilem + 1 + 15) & ~(size_t)15) + 64;
+    size_t off[8];
+    return work_layout(C->tilem, C->numtile, C->nnz, off);
 }
 
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
                        const tsg_spgemm_plan &P, void *work) {
-    long long *part1 = (long long *)work;
-    long long *part2 = part1 + tsg_scan_partial_count(C->tilem);
-    unsigned char *rowcls = (unsigned char *)(part2 + tsg_scan_partial_count(C->numtile));
-    int *cls_out = (int *)(rowcls + (((size_t)C->tilem + 1 + 15) & ~(size_t)15));
+    const SpgemmWork w = work_at(work, C->tilem, C->numtile, C->nnz, nullptr);
     TSG_TRY(launch_step1(s, A, B, C, false));
-    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, part1));
+    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, w.part1));
     TSG_TRY(launch_step1(s, A, B, C, true));
     const TileOperands op = make_operands(A, B);
-    TSG_TRY(launch_step2(s, op, C, P));
-    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, part2));
+    TSG_TRY(launch_step2(s, op, C, P, w));
+    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, w.part2));
     if (C->nnz > 0) {
-        if (!P.all) TSG_TRY(launch_row_class(s, op, C, rowcls, cls_out));
-        TSG_TRY(launch_step3(s, op, C, P, rowcls));
+        if (!P.all) TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
+        TSG_TRY(launch_step3(s, op, B, C, P, w));
     }
     return TSG_OK;
+}
+
+// ------------------------------------------------------------------ chunks
+// Row lengths of B from its tile masks: row 16K+r is the popcount of row r
+// over the tiles of tile-row K.
+__global__ void __launch_bounds__(256)
+k_tile_row_lengths(const int *__restrict__ Bptr, const unsigned short *__restrict__ Bmask, int m,
+                   int *__restrict__ len) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int K = i >> 4, r = i & 15;
+    int c = 0;
+    for (int t = Bptr[K]; t < Bptr[K + 1]; t++) c += __popc((unsigned)Bmask[(long long)t * 16 + r]);
+    len[i] = c;
+}
+
+// Intermediate products of every A tile-row: sum over A's nonzeros (i, k)
+// of B's row length k.  One wave per tile-row, lanes over its tiles.
+__global__ void __launch_bounds__(256)
+k_tile_row_products(const int *__restrict__ Aptr, const int *__restrict__ Acol, const unsigned short *__restrict__ Amask,
+                    int tilem, const int *__restrict__ blen, int bm, unsigned long long *__restrict__ out) {
+    const int I = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (I >= tilem) return;
+    unsigned long long s = 0;
+    for (int t = Aptr[I] + lane; t < Aptr[I + 1]; t += 64) {
+        const int k0 = Acol[t] * 16;
+        const uint4 *m4 = reinterpret_cast<const uint4 *>(Amask + (long long)t * 16);
+        const uint4 a = m4[0], b = m4[1];
+        const unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        int cc[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) cc[c] = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 16; c++) cc[c] += ((w[q] >> (15 - c)) & 1u) + ((w[q] >> (31 - c)) & 1u);
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+            if (cc[c] && k0 + c < bm) s += (unsigned long long)cc[c] * (unsigned long long)blen[k0 + c];
+    }
+    s = tsg_wave_sum(s);
+    if (lane == 0) out[I] = s;
+}
+
+int tsg_tile_row_products(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, unsigned long long *d_out) {
+    if (A->n != B->m) { tsg_set_error("tile_row_products: inner dimensions differ"); return TSG_ERR_ARG; }
+    int *blen;
+    TSG_TRY(tsg_alloc_t(ctx, &blen, (size_t)B->m + 1));
+    if (B->m > 0) {
+        hipLaunchKernelGGL(k_tile_row_lengths, dim3(tsg_div_up(B->m, 256)), dim3(256), 0, ctx->stream, B->tile_ptr,
+                           B->mask, B->m, blen);
+        TSG_LAUNCH_CHECK();
+    }
+    if (A->tilem > 0) {
+        hipLaunchKernelGGL(k_tile_row_products, dim3(tsg_div_up(A->tilem, 4)), dim3(256), 0, ctx->stream, A->tile_ptr,
+                           A->tile_colidx, A->mask, A->tilem, blen, B->m, d_out);
+        TSG_LAUNCH_CHECK();
+    }
+    tsg_free(ctx, blen);
+    return TSG_OK;
+}
+
+// Tile-row ranges of A whose intermediate products stay within
+// max_products each (a single tile-row above it forms its own chunk).
+// Every chunk's C then has at most max_products nonzeros.
+extern "C" int tsg_spgemm_chunk_plan(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, long long max_products,
+                                     int max_chunks, int *bounds, int *nchunks, unsigned long long *products_total) {
+    if (!A || !B || !bounds || !nchunks || max_chunks < 1 || max_products < 1) {
+        tsg_set_error("tsg_spgemm_chunk_plan: bad arguments");
+        return TSG_ERR_ARG;
+    }
+    unsigned long long *d;
+    TSG_TRY(tsg_alloc_t(ctx, &d, (size_t)A->tilem + 1));
+    TSG_TRY(tsg_tile_row_products(ctx, A, B, d));
+    std::vector<unsigned long long> h((size_t)A->tilem + 1);
+    if (A->tilem > 0)
+        TSG_HIP(hipMemcpyAsync(h.data(), d, 8 * (size_t)A->tilem, hipMemcpyDeviceToHost, ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    tsg_free(ctx, d);
+    int n = 0;
+    bounds[0] = 0;
+    unsigned long long acc = 0, tot = 0;
+    for (int I = 0; I < A->tilem; I++) {
+        tot += h[I];
+        if (acc > 0 && acc + h[I] > (unsigned long long)max_products) {
+            if (n + 1 >= max_chunks) {
+                tsg_set_error("tsg_spgemm_chunk_plan: more than %d chunks needed", max_chunks);
+                return TSG_ERR_ARG;
+            }
+            bounds[++n] = I;
+            acc = 0;
+        }
+        acc += h[I];
+    }
+    bounds[++n] = A->tilem;
+    *nchunks = n;
+    if (products_total) *products_total = tot;
+    return TSG_OK;
+}
+
+// C rows [16*r0, min(16*r1, m)) of A*B: A restricted to tile-rows [r0, r1)
+// as a view (tile_ptr offset into A's, every other array shared).
+extern "C" int tsg_spgemm_tile_rows(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, int r0, int r1,
+                                    const tsg_tune *tune, tsg_dtiles *C, tsg_stats *stats) {
+    if (!A || r0 < 0 || r1 < r0 || r1 > A->tilem) {
+        tsg_set_error("tsg_spgemm_tile_rows: bad tile-row range [%d, %d)", r0, r1);
+        return TSG_ERR_ARG;
+    }
+    tsg_dtiles v = *A;
+    v.tile_ptr = A->tile_ptr + r0;
+    v.tilem = r1 - r0;
+    v.m = std::min(16 * r1, A->m) - 16 * r0;
+    if (v.m < 0) v.m = 0;
+    return tsg_spgemm_planned(ctx, &v, B, tune, C, stats, nullptr);
 }

@@ -98,10 +98,14 @@ struct tsg_spgemm_plan {
     int s3_rows;    // some tile-row runs the row-wise step 3
     int s3_unfit;   // some tile-row runs the tile-parallel step 3
     int dense_min;  // dense accumulator threshold of the tile-parallel step 3
+    int s3_elem;    // the tile-rows the row-wise kernel cannot hold go element-wise
+    int elem_all;   // every tile-row goes element-wise
     int g3[6];      // row-wise step-3 LDS layout maxima
 };
 int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune,
                        tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out);
+// Intermediate products of each A tile-row (device u64[A->tilem]).
+int tsg_tile_row_products(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, unsigned long long *d_out);
 // Device scratch bytes tsg_spgemm_enqueue needs for this C.
 size_t tsg_spgemm_work_bytes(const tsg_dtiles *C);
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
@@ -109,6 +113,8 @@ int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, 
 
 // ------------------------------------------------------------------ conversions
 int tsg_tile2csr_view(tsg_ctx *ctx, const tsg_dtiles *t, int jbase, tsg_dcsr *out);
+// Block-diagonal packing of tile matrices on 16-row/column boundaries.
+int tsg_tiles_pack(tsg_ctx *ctx, const tsg_dtiles *const *parts, int count, tsg_dtiles *out);
 
 // ------------------------------------------------------------------ sort
 // Stable ascending sort of (key,value) pairs, 32-bit keys, using the

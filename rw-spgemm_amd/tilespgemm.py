@@ -62,7 +62,8 @@ class DTiles(C.Structure):
     _fields_ = [("m", _i), ("n", _i), ("tilem", _i), ("tilen", _i), ("numtile", _i), ("nnz", _i),
                 ("tile_ptr", _p), ("tile_colidx", _p), ("tile_rowidx", _p), ("tile_nnz", _p),
                 ("csr_ptr", _p), ("csr_col", _p), ("val", _p), ("mask", _p),
-                ("csc_ptr", _p), ("csc_rowidx", _p), ("csc_tileid", _p)]
+                ("csc_ptr", _p), ("csc_rowidx", _p), ("csc_tileid", _p), ("rowx_ptr", _p), ("rowx_col", _p),
+                ("rowx_val", _p)]
 
 
 class HTiles(C.Structure):
@@ -77,7 +78,7 @@ class HTiles(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("force_tile_parallel", _i),
-                ("batch_group_nnz_cap", _i), ("reserved", _i * 4)]
+                ("batch_group_nnz_cap", _i), ("element_mode", _i), ("reserved", _i * 3)]
 
 
 class Stats(C.Structure):
@@ -112,6 +113,25 @@ class BatchSpec(C.Structure):
 OP_A2, OP_AAT = 0, 1
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", _ll), ("slots", _i), ("max_batch", _i), ("small_bytes", _ll),
+                ("chunk_products", _ll), ("starve_ms", _d), ("tune", Tune)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", _i), ("handle", _i), ("op", _i), ("chunks", _i), ("fused_with", _i),
+                ("numtileC", _ll), ("nnzC", _ll), ("bytes", _ll), ("t_arrival_ms", _d), ("t_admit_ms", _d),
+                ("t_done_ms", _d)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("resident_bytes", _ll), ("reserved_bytes", _ll), ("peak_reserved_bytes", _ll), ("queued", _i),
+                ("fused_launches", _ll), ("single_launches", _ll)]
+
+
 def _sig(name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -143,6 +163,7 @@ _sig("tsg_dcsr_transpose", _i, [_p, C.POINTER(DCsr), C.POINTER(DCsr)])
 _sig("tsg_dcsr_nnzcub", _i, [_p, C.POINTER(DCsr), C.POINTER(DCsr), C.POINTER(_ull)])
 _sig("tsg_csr2tile", _i, [_p, C.POINTER(DCsr), C.POINTER(DTiles)])
 _sig("tsg_tiles_build_csc", _i, [_p, C.POINTER(DTiles)])
+_sig("tsg_tiles_build_rowindex", _i, [_p, C.POINTER(DTiles)])
 _sig("tsg_tile2csr", _i, [_p, C.POINTER(DTiles), C.POINTER(DCsr)])
 _sig("tsg_tiles_free", None, [_p, C.POINTER(DTiles)])
 _sig("tsg_tiles_download", _i, [_p, C.POINTER(DTiles), C.POINTER(HTiles)])
@@ -177,12 +198,28 @@ _sig("tilespgemm", None, [C.POINTER(SMatrix), C.POINTER(SMatrix), C.POINTER(SMat
                           _ull, C.POINTER(_ull), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.c_char_p,
                           C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)])
 _sig("tsg_nnzcub", _ull, [C.POINTER(SMatrix), C.POINTER(SMatrix)])
+_sig("tsg_spgemm_chunk_plan", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), _ll, _i, C.POINTER(_i), C.POINTER(_i),
+                                   C.POINTER(_ull)])
+_sig("tsg_spgemm_tile_rows", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), _i, _i, C.POINTER(Tune),
+                                  C.POINTER(DTiles), C.POINTER(Stats)])
 _sig("tsg_batch_create", _i, [_p, C.POINTER(DCsr), C.POINTER(_i), _i, C.POINTER(Tune), C.POINTER(_p)])
 _sig("tsg_batch_run", _i, [_p])
 _sig("tsg_batch_autotune", _i, [_p, _i, C.POINTER(Tune), C.POINTER(_d)])
 _sig("tsg_batch_get_info", _i, [_p, C.POINTER(BatchInfo)])
 _sig("tsg_batch_result", _i, [_p, _i, C.POINTER(HTiles), C.POINTER(HCsr)])
 _sig("tsg_batch_destroy", None, [_p])
+_sig("tsg_server_config_default", None, [C.POINTER(ServerConfig)])
+_sig("tsg_server_create", _i, [_i, C.POINTER(ServerConfig), C.POINTER(_p)])
+_sig("tsg_server_destroy", None, [_p])
+_sig("tsg_server_register", _i, [_p, C.POINTER(HCsr), C.POINTER(_i)])
+_sig("tsg_server_size", _i, [_p, _i, _i, C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_ll), C.POINTER(_i)])
+_sig("tsg_server_submit", _i, [_p, _i, _i, C.POINTER(_ull)])
+_sig("tsg_server_wait", _i, [_p, _ull, _d, C.POINTER(RequestInfo)])
+_sig("tsg_server_result_csr", _i, [_p, _ull, C.POINTER(HCsr)])
+_sig("tsg_server_release", _i, [_p, _ull])
+_sig("tsg_server_stats", _i, [_p, C.POINTER(ServerStats)])
+_sig("tsg_server_replay", _i, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), _i, C.POINTER(_ull),
+                               C.POINTER(RequestInfo)])
 _sig("tsg_gen_batch_spec", _i, [_i, _ull, _i, _i, C.POINTER(BatchSpec)])
 _sig("tsg_gen_batch_problem", _i, [C.POINTER(BatchSpec), C.POINTER(HCsr)])
 _sig("tsg_compat_set_device", _i, [_i])
@@ -321,6 +358,72 @@ class Batch:
             self.h = None
 
 
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    lib.tsg_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching SpGEMM server (tsg_server) on one device."""
+
+    def __init__(self, device=0, config: ServerConfig | None = None):
+        p = _p()
+        _check(lib.tsg_server_create(device, C.byref(config) if config else None, C.byref(p)))
+        self.h = p
+
+    def register(self, csr: Csr) -> int:
+        h = csr.as_c()
+        out = _i()
+        _check(lib.tsg_server_register(self.h, C.byref(h), C.byref(out)))
+        return out.value
+
+    def size(self, handle, op):
+        t, n, b, c = _ll(), _ll(), _ll(), _i()
+        _check(lib.tsg_server_size(self.h, handle, op, C.byref(t), C.byref(n), C.byref(b), C.byref(c)))
+        return {"numtileC": t.value, "nnzC": n.value, "bytes": b.value, "chunks": c.value}
+
+    def submit(self, handle, op) -> int:
+        rid = _ull()
+        _check(lib.tsg_server_submit(self.h, handle, op, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout_ms=0.0) -> RequestInfo:
+        info = RequestInfo()
+        _check(lib.tsg_server_wait(self.h, rid, timeout_ms, C.byref(info)))
+        return info
+
+    def result_csr(self, rid) -> Csr:
+        h = HCsr()
+        _check(lib.tsg_server_result_csr(self.h, rid, C.byref(h)))
+        return _take_hcsr(h)
+
+    def release(self, rid):
+        _check(lib.tsg_server_release(self.h, rid))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib.tsg_server_stats(self.h, C.byref(st)))
+        return st
+
+    def replay(self, handles, ops, arrival_ms, keep=False):
+        n = len(handles)
+        hs = (_i * n)(*handles)
+        os_ = (_i * n)(*ops)
+        ar = (_d * n)(*arrival_ms)
+        ids = (_ull * n)()
+        infos = (RequestInfo * n)()
+        _check(lib.tsg_server_replay(self.h, n, hs, os_, ar, 1 if keep else 0, ids, infos))
+        return [ids[i] for i in range(n)], [infos[i] for i in range(n)]
+
+    def close(self):
+        if self.h:
+            lib.tsg_server_destroy(self.h)
+            self.h = None
+
+
 def gen_stencil5(nx, ny):
     h = HCsr()
     _check(lib.tsg_gen_stencil5(nx, ny, C.byref(h)))
@@ -435,6 +538,10 @@ class Context:
         _check(lib.tsg_csr2tile(self.h, C.byref(d), C.byref(t)))
         return t
 
+    def build_rowindex(self, t: DTiles):
+        _check(lib.tsg_tiles_build_rowindex(self.h, C.byref(t)))
+        return t
+
     def build_csc(self, t: DTiles):
         _check(lib.tsg_tiles_build_csc(self.h, C.byref(t)))
         return t
@@ -547,6 +654,23 @@ class Context:
         finally:
             self.dfree(pr)
             self.dfree(pc)
+
+    def chunk_plan(self, a: DTiles, b: DTiles, max_products: int, max_chunks: int = 4096):
+        """Tile-row bounds of chunks with at most max_products products each;
+        returns (bounds list, total products)."""
+        bounds = (_i * (max_chunks + 1))()
+        n = _i()
+        tot = _ull()
+        _check(lib.tsg_spgemm_chunk_plan(self.h, C.byref(a), C.byref(b), max_products, max_chunks, bounds,
+                                         C.byref(n), C.byref(tot)))
+        return [bounds[i] for i in range(n.value + 1)], tot.value
+
+    def spgemm_tile_rows(self, a: DTiles, b: DTiles, r0: int, r1: int, tune: Tune | None = None, timed=True):
+        c = DTiles()
+        st = Stats()
+        _check(lib.tsg_spgemm_tile_rows(self.h, C.byref(a), C.byref(b), r0, r1, C.byref(tune) if tune else None,
+                                        C.byref(c), C.byref(st) if timed else None))
+        return c, st
 
     def spgemm(self, a: DTiles, b: DTiles, tune: Tune | None = None, timed=True):
         c = DTiles()

@@ -76,6 +76,17 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
         assert np.array_equal(C2[k], C[k]), f"tile-parallel path differs in {k}"
     ctx.free(tc2)
+    # element-wise numeric step with B's row index: for every tile-row, and
+    # for the tile-rows the row-wise tile kernel cannot hold
+    ctx.build_rowindex(tb)
+    for mode in (2, 0):
+        t3 = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
+        t3.element_mode = mode
+        tc3, _ = ctx.spgemm(ta, tb, t3)
+        C3 = ctx.download_tiles(tc3)
+        for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
+            assert np.array_equal(C3[k], C[k]), f"element-wise path (mode {mode}) differs in {k}"
+        ctx.free(tc3)
     for obj in (dc, tc):
         ctx.free(obj)
     ctx.free(ta)
