@@ -242,6 +242,65 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
             "resident_gib": round(st1.resident_bytes / 2 ** 30, 2)}
 
 
+def rmat_leg(ts, scale=20, chunk_products=1500000000):
+    """R-MAT scale-20 C = A*A^T (edge factor 16, (0.57, 0.19, 0.19, 0.05),
+    symmetrized, duplicates merged, values U(0,1], seed 1) under the random,
+    RCM and degree orderings, each computed on the device.  C (~2.5e10
+    nonzeros) does not fit next to its operands, so it runs in tile-row
+    chunks of at most chunk_products products; each chunk is computed and
+    dropped.  GFLOPS = 2 * nnzCub / (sum of chunk pipeline times)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    a = ts.gen_rmat(scale, 16, seed=1)
+    progress(f"rmat{scale}: generated {a.nnz} nnz in {time.perf_counter() - t0:.1f} s")
+    ctx = ts.Context(0)
+    d = ctx.upload(a)
+    out = {"scale": scale, "n": a.m, "nnz": a.nnz, "orders": {}}
+    nnzc = set()
+    for algo in ("random", "rcm", "degree"):
+        ctx.sync()
+        t0 = time.perf_counter()
+        if algo == "rcm":
+            old, _ = ctx.order_rcm(d)
+        elif algo == "degree":
+            old = ctx.order_degree(d)
+        else:
+            old = ctx.order_random(a.m, 1)
+        t_order = time.perf_counter() - t0
+        new = np.empty_like(old)
+        new[old] = np.arange(a.m, dtype=np.int32)
+        t0 = time.perf_counter()
+        dp = ctx.permute(d, old, new)
+        ta = ctx.csr2tile(dp)
+        ctx.build_csc(ta)
+        dpt = ctx.transpose(dp)
+        tb = ctx.csr2tile(dpt)
+        ctx.build_csc(tb)
+        ctx.build_rowindex(tb)
+        ctx.sync()
+        t_prep = time.perf_counter() - t0
+        bounds, prods = ctx.chunk_plan(ta, tb, chunk_products)
+        tot_ms, nnz, tiles = 0.0, 0, 0
+        for r0, r1 in zip(bounds, bounds[1:]):
+            c, st = ctx.spgemm_tile_rows(ta, tb, r0, r1)
+            tot_ms += st.t_total_ms
+            nnz += st.nnzC
+            tiles += st.numtileC
+            ctx.free(c)
+        nnzc.add(nnz)
+        out["orders"][algo] = {"order_s": round(t_order, 3), "prep_s": round(t_prep, 3), "A_tiles": ta.numtile,
+                               "chunks": len(bounds) - 1, "nnzCub": prods, "nnzC": nnz, "tilesC": tiles,
+                               "spgemm_ms": round(tot_ms, 2), "gflops": round(2.0 * prods / (tot_ms * 1e6), 2)}
+        progress(f"rmat{scale} {algo}: {out['orders'][algo]}")
+        for o in (tb, dpt, ta, dp):
+            ctx.free(o)
+        ctx.trim()
+    out["nnzC_invariant_across_orders"] = len(nnzc) == 1
+    ctx.free(d)
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +312,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--rmat-only", action="store_true")
+    ap.add_argument("--rmat-scale", type=int, default=20)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rmat-scale", type=int, default=18)
@@ -267,6 +328,9 @@ def main():
     import numpy as np
     import tilespgemm as ts
 
+    if args.rmat_only:
+        print(json.dumps({"rmat": rmat_leg(ts, scale=args.rmat_scale)}), flush=True)
+        return
     if args.serve_only:
         print(json.dumps({"served": serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)}),
               flush=True)

@@ -394,6 +394,22 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
 
 // Tile-parallel step 2 for tile-rows with more than G2_UMAX C tiles: one C
 // tile per 16-lane group, pairs from the sorted-list intersection.
+// Tile-parallel grids cover 16 work-items per C tile, which exceeds the
+// 2^32 work-items of a launch beyond 2^28 tiles: such kernels run at most
+// TP_MAX_BLOCKS blocks, each looping over virtual blocks.
+#define TP_MAX_BLOCKS (1u << 20)
+
+static inline unsigned tp_grid(long long vblocks) {
+    return (unsigned)std::min<long long>(std::max<long long>(vblocks, 1), TP_MAX_BLOCKS);
+}
+
+__device__ __forceinline__ void step2_tile_body(const TileOperands &op, const int *__restrict__ Cptr,
+                                                const int *__restrict__ Crow, const int *__restrict__ Ccol,
+                                                int numtileC, unsigned short *__restrict__ Cmask,
+                                                unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz,
+                                                int all_rows, long long vblk, unsigned short *s_bm,
+                                                int (*s_la)[PAIR_LDS_MAX], int (*s_lb)[PAIR_LDS_MAX]);
+
 __global__ void __launch_bounds__(64 * S23_WAVES)
 k_step2_tiles(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Crow,
               const int *__restrict__ Ccol, int numtileC, unsigned short *__restrict__ Cmask,
@@ -401,16 +417,28 @@ k_step2_tiles(TileOperands op, const int *__restrict__ Cptr, const int *__restri
     __shared__ unsigned short s_bm[S23_WAVES][64];
     __shared__ int s_la[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
     __shared__ int s_lb[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
+    const int wave = threadIdx.x >> 6;
+    const long long nvb = ((long long)numtileC + S23_WAVES * TILES_PER_WAVE - 1) / (S23_WAVES * TILES_PER_WAVE);
+    for (long long vb = tsg_xcd_block(blockIdx.x, gridDim.x); vb < nvb; vb += gridDim.x)
+        step2_tile_body(op, Cptr, Crow, Ccol, numtileC, Cmask, Cptr16, Cnnz, all_rows, vb, s_bm[wave], s_la[wave],
+                        s_lb[wave]);
+}
+
+__device__ __forceinline__ void step2_tile_body(const TileOperands &op, const int *__restrict__ Cptr,
+                                                const int *__restrict__ Crow, const int *__restrict__ Ccol,
+                                                int numtileC, unsigned short *__restrict__ Cmask,
+                                                unsigned char *__restrict__ Cptr16, int *__restrict__ Cnnz,
+                                                int all_rows, long long vblk, unsigned short *s_bmw,
+                                                int (*s_law)[PAIR_LDS_MAX], int (*s_lbw)[PAIR_LDS_MAX]) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const unsigned lblk = tsg_xcd_block(blockIdx.x, gridDim.x);
-    const long long t = ((long long)lblk * S23_WAVES + wave) * TILES_PER_WAVE + g;
+    const long long t = (vblk * S23_WAVES + wave) * TILES_PER_WAVE + g;
     if (t >= numtileC) return;  // whole groups exit together
     const int I = Crow[t], J = Ccol[t];
     if (!all_rows && Cptr[I + 1] - Cptr[I] <= G2_UMAX) return;  // done by k_step2_rows
-    unsigned short *bmg = &s_bm[wave][g * 16];
+    unsigned short *bmg = &s_bmw[g * 16];
     unsigned orow = 0;
-    for_each_pair(op, I, J, g, r, s_la[wave][g], s_lb[wave][g], [&](int ta, int tb) {
+    for_each_pair(op, I, J, g, r, s_law[g], s_lbw[g], [&](int ta, int tb) {
         bmg[r] = op.Bmask[(long long)tb * 16 + r];
         tsg_wave_sync();
         unsigned am = op.Amask[(long long)ta * 16 + r];
@@ -480,6 +508,16 @@ k_max_rowlen(const int *__restrict__ ptr, int n, int *__restrict__ out) {
 
 #define ACC_STRIDE 17  // doubles per lane row (padding against bank conflicts)
 
+__device__ __forceinline__ void step3_tile_body(const TileOperands &op, const int *__restrict__ Crow,
+                                                const int *__restrict__ Ccol, int numtileC,
+                                                const unsigned short *__restrict__ Cmask,
+                                                const unsigned char *__restrict__ Cptr16,
+                                                const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
+                                                double *__restrict__ Cval, int dense_min, int all_rows,
+                                                const unsigned char *__restrict__ rowcls, long long vblk,
+                                                double *s_accw, unsigned short *s_bmw, unsigned char *s_bpw,
+                                                int (*s_law)[PAIR_LDS_MAX], int (*s_lbw)[PAIR_LDS_MAX]);
+
 __global__ void __launch_bounds__(64 * S23_WAVES)
 k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Crow, const int *__restrict__ Ccol,
         int numtileC, const unsigned short *__restrict__ Cmask, const unsigned char *__restrict__ Cptr16,
@@ -490,10 +528,25 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
     __shared__ unsigned char s_bp[S23_WAVES][64];
     __shared__ int s_la[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
     __shared__ int s_lb[S23_WAVES][TILES_PER_WAVE][PAIR_LDS_MAX];
+    const int wave = threadIdx.x >> 6;
+    const long long nvb = ((long long)numtileC + S23_WAVES * TILES_PER_WAVE - 1) / (S23_WAVES * TILES_PER_WAVE);
+    for (long long vb = tsg_xcd_block(blockIdx.x, gridDim.x); vb < nvb; vb += gridDim.x)
+        step3_tile_body(op, Crow, Ccol, numtileC, Cmask, Cptr16, Cnnz, Ccol8, Cval, dense_min, all_rows, rowcls, vb,
+                        s_acc[wave], s_bm[wave], s_bp[wave], s_la[wave], s_lb[wave]);
+}
+
+__device__ __forceinline__ void step3_tile_body(const TileOperands &op, const int *__restrict__ Crow,
+                                                const int *__restrict__ Ccol, int numtileC,
+                                                const unsigned short *__restrict__ Cmask,
+                                                const unsigned char *__restrict__ Cptr16,
+                                                const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
+                                                double *__restrict__ Cval, int dense_min, int all_rows,
+                                                const unsigned char *__restrict__ rowcls, long long vblk,
+                                                double *s_accw, unsigned short *s_bmw, unsigned char *s_bpw,
+                                                int (*s_law)[PAIR_LDS_MAX], int (*s_lbw)[PAIR_LDS_MAX]) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const unsigned lblk = tsg_xcd_block(blockIdx.x, gridDim.x);
-    const long long t = ((long long)lblk * S23_WAVES + wave) * TILES_PER_WAVE + g;
+    const long long t = (vblk * S23_WAVES + wave) * TILES_PER_WAVE + g;
     if (t >= numtileC) return;
     const int coff = Cnnz[t], cn = Cnnz[t + 1] - coff;
     if (cn == 0) return;  // structurally empty tile (step-1 phantom)
@@ -502,18 +555,18 @@ k_step3(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ C
     const int cp = Cptr16[t * 16 + r];
     const int rowlen = __popc(cm);
     const bool dense = cn >= dense_min;
-    double *accg = &s_acc[wave][g * 16 * ACC_STRIDE];  // 272 doubles per group
-    double *accr = accg + r * ACC_STRIDE;               // dense: lane-private row
+    double *accg = &s_accw[g * 16 * ACC_STRIDE];  // 272 doubles per group
+    double *accr = accg + r * ACC_STRIDE;          // dense: lane-private row
     if (dense) {
 #pragma unroll
         for (int c = 0; c < 16; c++) accr[c] = 0.0;
     } else {
         for (int j = 0; j < rowlen; j++) accg[cp + j] = 0.0;
     }
-    unsigned short *bmg = &s_bm[wave][g * 16];
-    unsigned char *bpg = &s_bp[wave][g * 16];
+    unsigned short *bmg = &s_bmw[g * 16];
+    unsigned char *bpg = &s_bpw[g * 16];
     const int I = Crow[t], J = Ccol[t];
-    for_each_pair(op, I, J, g, r, s_la[wave][g], s_lb[wave][g], [&](int ta, int tb) {
+    for_each_pair(op, I, J, g, r, s_law[g], s_lbw[g], [&](int ta, int tb) {
         bmg[r] = op.Bmask[(long long)tb * 16 + r];
         bpg[r] = op.Bptr16[(long long)tb * 16 + r];
         tsg_wave_sync();
@@ -1122,7 +1175,7 @@ static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, co
                         const SpgemmWork &w) {
     if (C->numtile <= 0) return TSG_OK;
     if (P.all) {
-        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
+        const unsigned nblk = tp_grid(tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE));
         hipLaunchKernelGGL(k_step2_tiles, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
                            C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, 1);
         TSG_LAUNCH_CHECK();
@@ -1168,7 +1221,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
                            C->tile_nnz, C->csr_col, C->val);
         TSG_LAUNCH_CHECK();
     } else if (P.all || P.s3_unfit) {
-        const unsigned nblk = tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE);
+        const unsigned nblk = tp_grid(tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE));
         hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
                            C->tile_colidx, C->numtile, C->mask, C->csr_ptr, C->tile_nnz, C->csr_col, C->val,
                            P.dense_min, P.all, (const unsigned char *)w.rowcls);

@@ -104,11 +104,12 @@ def test_batch_groups_split(ctx):
     """A small per-group nonzero cap forces several fused groups inside the
     one graph; results are unchanged."""
     mats, ops = spec_mats(12, 7, 2000)
+    sizes = [plain_tiles(ctx, a, op)["nnz"] for a, op in zip(mats, ops)]
     tune = ts.default_tune()
-    tune.batch_group_nnz_cap = 150000
+    tune.batch_group_nnz_cap = max(sizes)  # every problem fits alone, not all together
     info = run_batch(ctx, mats, ops, tune=tune)
-    log(f"[batch] groups {info.groups}")
-    assert info.groups > 1
+    log(f"[batch] groups {info.groups} (cap {max(sizes)}, total {sum(sizes)})")
+    assert info.groups > 1 and info.nnzC == sum(sizes)
 
 
 def test_batch_autotune_same_results(ctx):

@@ -85,7 +85,7 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
         tc3, _ = ctx.spgemm(ta, tb, t3)
         C3 = ctx.download_tiles(tc3)
         for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
-            assert np.array_equal(C3[k], C[k]), f"element-wise path (mode {mode}) differs in {k}"
+            assert np.array_equal(C3[k], C[k]), f"element mode {mode} differs in {k}"
         ctx.free(tc3)
     for obj in (dc, tc):
         ctx.free(obj)
