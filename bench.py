@@ -70,7 +70,7 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotune=True):
+def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotune=True, sequential=True):
     """Batched workload: `count` SpGEMMs (A^2 / AA^T, 50/50; n log-uniform in
     [nmin, nmax]; Erdos-Renyi mean nnz/row U[4,32] or banded half-bandwidth
     U{2..40}; values U[-1,1)).  Batched = one graph launch per step over
@@ -85,29 +85,34 @@ def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotu
     dps = [ctx.upload(m) for m in mats]
     nnz_total = sum(m.nnz for m in mats)
     del mats
-    # plain operands for the sequential calls
+    # plain operands for the sequential calls, prepared like the batch's
+    # (CSC and row index on B), so both run the same kernels
     seq = []
     for d, op in zip(dps, ops):
         ta = ctx.csr2tile(d)
         ctx.build_csc(ta)
         if op == ts.OP_A2:
+            ctx.build_rowindex(ta)
             seq.append((ta, ta, None))
         else:
             dt = ctx.transpose(d)
             tb = ctx.csr2tile(dt)
             ctx.build_csc(tb)
+            ctx.build_rowindex(tb)
             ctx.free(dt)
             seq.append((ta, tb, tb))
-    for ta, tb, _ in seq:  # warm the allocator cache
-        tc, _ = ctx.spgemm(ta, tb, timed=False)
-        ctx.free(tc)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for ta, tb, _ in seq:
-        tc, _ = ctx.spgemm(ta, tb, timed=False)
-        ctx.free(tc)
-    ctx.sync()
-    seq_s = time.perf_counter() - t0
+    seq_s = float("nan")
+    if sequential:
+        for ta, tb, _ in seq:  # warm the allocator cache
+            tc, _ = ctx.spgemm(ta, tb, timed=False)
+            ctx.free(tc)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for ta, tb, _ in seq:
+            tc, _ = ctx.spgemm(ta, tb, timed=False)
+            ctx.free(tc)
+        ctx.sync()
+        seq_s = time.perf_counter() - t0
     progress(f"batch: sequential {seq_s * 1e3:.1f} ms")
     for ta, tb, own in seq:
         ctx.free(ta)
@@ -311,6 +316,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-no-sequential", action="store_true")
+    ap.add_argument("--element-mode", type=int, default=None,
+                    help="build B's row index and use this element_mode (0 auto, 1 never, 2 every tile-row)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--rmat-only", action="store_true")
     ap.add_argument("--rmat-scale", type=int, default=20)
@@ -337,7 +346,8 @@ def main():
         return
     if args.batch_only:
         ctx = ts.Context(0)
-        print(json.dumps({"batched": batch_leg(ts, ctx, count=args.batch_count)}), flush=True)
+        print(json.dumps({"batched": batch_leg(ts, ctx, count=args.batch_count, autotune=not args.no_autotune,
+                                               sequential=not args.batch_no_sequential)}), flush=True)
         ctx.close()
         return
     N = args.grid
@@ -350,6 +360,9 @@ def main():
     ctx.build_csc(ta)
     nnzcub = ctx.nnzcub(da, da)
     tune = ts.default_tune()
+    if args.element_mode is not None:
+        ctx.build_rowindex(ta)
+        tune.element_mode = args.element_mode
     if args.dense_threshold:
         tune.dense_acc_min_nnz = args.dense_threshold
     for _ in range(args.warmup):
@@ -408,7 +421,7 @@ def main():
     ctx.free(da)
     if not args.no_batch:
         ctx.trim()
-        out["batched"] = batch_leg(ts, ctx, count=args.batch_count)
+        out["batched"] = batch_leg(ts, ctx, count=args.batch_count, autotune=not args.no_autotune)
     ctx.close()
     if not args.no_serve:
         out["served"] = serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)

@@ -407,10 +407,14 @@ extern "C" int tsg_batch_autotune(tsg_batch *b, int reps, tsg_tune *chosen, doub
     // element-wise numeric step for the rows the row-wise kernel cannot
     // hold, or for every row; the tile-parallel kernel with several
     // dense-accumulator thresholds for those rows, or for every row
+    long long tiles = 0;
+    for (const auto &g : b->groups) tiles += g.plan.numtileC;
     add(0, 0, 64);
     add(0, 2, 64);
-    for (int th : {16, 64, 160, 257}) add(0, 1, th);
-    for (int th : {64, 257}) add(1, 1, th);
+    for (int th : {16, 64, 257}) add(0, 1, th);
+    // every tile-row tile-parallel only while that stays affordable to try
+    if (tiles < (64LL << 20))
+        for (int th : {64, 257}) add(1, 1, th);
     double best = 1e30;
     tsg_tune bt = b->tune;
     for (const auto &t : cands) {

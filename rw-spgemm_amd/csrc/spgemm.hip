@@ -288,8 +288,11 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     __shared__ int s_tot;
     __shared__ unsigned short s_bst[G2_THREADS][16];
     const int tid = threadIdx.x;
-    const int w = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
-    if (w >= *nseg) return;  // block-uniform
+    // the grid is an upper bound: the first *nseg blocks (spread evenly over
+    // the XCDs) take the segments, each XCD a contiguous range
+    const int ns = *nseg;
+    if ((int)blockIdx.x >= ns) return;  // block-uniform
+    const int w = (int)tsg_xcd_block(blockIdx.x, (unsigned)ns);
     const int4 sg = segs[w];
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
@@ -618,6 +621,9 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // C entry again sums its products in the serial SPA order, without atomics.
 #define G3E_UMAX 256
 #define G3E_NNZ 2048
+#define G3E_HASH 512  // open-addressing table: C tile column -> slot (load <= 1/2)
+
+__device__ __forceinline__ unsigned col_hash(int J) { return ((unsigned)J * 2654435761u) >> 23; }  // 9 bits
 
 __global__ void __launch_bounds__(256)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
@@ -630,16 +636,27 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     __shared__ unsigned short s_cmask[G3E_UMAX * 16];
     __shared__ unsigned char s_cptr[G3E_UMAX * 16];
     __shared__ double s_acc[G3E_NNZ];
+    __shared__ int s_hkey[G3E_HASH];
+    __shared__ unsigned char s_hval[G3E_HASH];
     const int tid = threadIdx.x;
-    const int w = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
-    if (w >= *nseg) return;  // block-uniform
+    // the grid is an upper bound: the first *nseg blocks (spread evenly over
+    // the XCDs) take the segments, each XCD a contiguous range
+    const int ns = *nseg;
+    if ((int)blockIdx.x >= ns) return;  // block-uniform
+    const int w = (int)tsg_xcd_block(blockIdx.x, (unsigned)ns);
     const int4 sg = segs[w];
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     const int nbase = Cnnz[c0], nseg_nnz = Cnnz[c0 + u] - nbase;
     for (int i = tid; i < nseg_nnz; i += 256) s_acc[i] = 0.0;
+    for (int i = tid; i < G3E_HASH; i += 256) s_hkey[i] = -1;
+    __syncthreads();
     for (int i = tid; i < u; i += 256) {
-        s_ccol[i] = Ccol[c0 + i];
+        const int J = Ccol[c0 + i];
+        unsigned h = col_hash(J);
+        while (atomicCAS(&s_hkey[h], -1, J) != -1) h = (h + 1) & (G3E_HASH - 1);  // u <= G3E_HASH / 2
+        s_hval[h] = (unsigned char)i;
+        s_ccol[i] = J;
         s_coff[i] = Cnnz[c0 + i] - nbase;
         const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
         reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
@@ -650,35 +667,60 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
     const int r = tid >> 4, l = tid & 15;
     const int row = 16 * I + r;
+    const int gb = tid & 48;  // first lane of this group in its wave
     if (row < m && nseg_nnz > 0) {
         const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-        for (int t = a0; t < a1; t++) {
-            unsigned am = op.Amask[(long long)t * 16 + r];
-            if (!am) continue;
-            const double *av = op.Aval + op.Annz[t] + op.Aptr16[(long long)t * 16 + r];
-            const int k16 = op.Acol[t] * 16;
-            int ia = 0;
-            while (am) {
-                const int b = 31 - __clz(am);
-                am &= ~(1u << b);
-                const int k = k16 + 15 - b;
-                const double a = av[ia++];
-                const int e1 = Browp[k + 1];
-                const int e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
-                for (int e = e0 + l; e < e1; e += 16) {
-                    const int col = Bcolx[e];
-                    if (col >= col_hi) break;
-                    const int J = col >> 4, c = col & 15;
-                    int lo = 0, hi = u - 1;  // J is in the segment's tile list
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+        // the 16 lanes load 16 A tiles' row masks at once; the tiles with an
+        // entry in row r are then taken one by one in ascending K
+        for (int t0 = a0; t0 < a1; t0 += 16) {
+            const int tl = t0 + l;
+            const unsigned am_l = tl < a1 ? op.Amask[(long long)tl * 16 + r] : 0u;
+            unsigned nz = (unsigned)(__ballot(am_l != 0) >> gb) & 0xffffu;
+            while (nz) {
+                const int src = __ffs(nz) - 1;
+                nz &= nz - 1;
+                const int t = t0 + src;
+                unsigned am = (unsigned)__shfl((int)am_l, gb + src, 64);
+                const double *av = op.Aval + op.Annz[t] + op.Aptr16[(long long)t * 16 + r];
+                const int k16 = op.Acol[t] * 16;
+                int ia = 0;
+                while (am) {
+                    const int b = 31 - __clz(am);
+                    am &= ~(1u << b);
+                    const int k = k16 + 15 - b;
+                    const double a = av[ia++];
+                    const int e1 = Browp[k + 1];
+                    // first entry of B row k at or after col_lo: 16-ary search,
+                    // one probe per lane per round
+                    int lo = Browp[k], hi = e1;  // answer in [lo, hi]
+                    while (hi - lo > 1) {
+                        const int step = (hi - lo + 15) >> 4;
+                        const int pos = lo + l * step;
+                        const bool ge = pos < hi && Bcolx[pos] >= col_lo;
+                        const unsigned gm = (unsigned)(__ballot(ge) >> gb) & 0xffffu;
+                        const int f = gm ? __ffs(gm) - 1 : 16;  // first probe at or past col_lo
+                        if (f == 0) { hi = lo; break; }
+                        // the answer follows the last probe before col_lo
+                        const int last = f < 16 ? f - 1 : min(15, (hi - lo - 1) / step);
+                        if (f < 16) hi = lo + f * step;
+                        lo = lo + last * step + 1;
                     }
-                    const unsigned cm = s_cmask[lo * 16 + r];
-                    double *acc = &s_acc[s_coff[lo] + s_cptr[lo * 16 + r] + __popc(cm >> (16 - c))];
-                    *acc = __builtin_fma(a, Bvalx[e], *acc);
+                    const int e0 = (hi > lo && Bcolx[lo] >= col_lo) ? lo : hi;
+                    for (int e = e0 + l; e < e1; e += 16) {
+                        const int col = Bcolx[e];
+                        if (col >= col_hi) break;
+                        const int J = col >> 4, c = col & 15;
+                        unsigned h = col_hash(J);  // J is in the segment's tile list
+                        int probe = 0;
+                        while (s_hkey[h] != J && ++probe < G3E_HASH) h = (h + 1) & (G3E_HASH - 1);
+                        if (probe == G3E_HASH) continue;  // not reached for a consistent C structure
+                        const int lo2 = s_hval[h];
+                        const unsigned cm = s_cmask[lo2 * 16 + r];
+                        double *acc = &s_acc[s_coff[lo2] + s_cptr[lo2 * 16 + r] + __popc(cm >> (16 - c))];
+                        *acc = __builtin_fma(a, Bvalx[e], *acc);
+                    }
+                    tsg_wave_sync();  // the next k of this row reads what lanes wrote
                 }
-                tsg_wave_sync();  // the next k of this row reads what lanes wrote
             }
         }
     }
