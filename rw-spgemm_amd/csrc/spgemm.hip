@@ -206,11 +206,11 @@ __device__ __forceinline__ void for_each_pair(const TileOperands &op, int I, int
 // tile-rows with rowcls == want (want < 0: all) are cut.
 template <int MODE>
 __device__ __forceinline__ int seg_walk(const int *__restrict__ Cnnz, int c0, int u, int max_tiles, int max_nnz,
-                                        int4 *out, int I) {
+                                        int4 *out, int I, int kind = 0) {
     if (MODE == 0) {
         const int n = (u + max_tiles - 1) / max_tiles;
         if (out)
-            for (int j = 0; j < n; j++) out[j] = make_int4(I, j * max_tiles, min(u, (j + 1) * max_tiles), 0);
+            for (int j = 0; j < n; j++) out[j] = make_int4(I, j * max_tiles, min(u, (j + 1) * max_tiles), kind);
         return n;
     }
     int n = 0, s0 = 0, acc = 0;
@@ -238,12 +238,27 @@ k_seg(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem, con
     const int I = blockIdx.x * 256 + threadIdx.x;
     if (I >= tilem) return;
     const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
+    // MODE 0 carries rowcls (the step-2 kernel kind) into each item's .w
     const bool take = u > 0 && (want < 0 || rowcls[I] == want);
+    const int kind = (MODE == 0 && rowcls) ? rowcls[I] : 0;
     if (!items) {
-        cnt_or_off[I] = take ? seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, nullptr, I) : 0;
+        cnt_or_off[I] = take ? seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, nullptr, I, kind) : 0;
     } else if (take) {
-        seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, items + cnt_or_off[I], I);
+        seg_walk<MODE>(Cnnz, c0, u, max_tiles, max_nnz, items + cnt_or_off[I], I, kind);
     }
+}
+
+// Step-2 kernel kind of every tile-row: 1 = element-wise (B has its row
+// index and A's tile-row is hypersparse: at most 4 entries per tile on
+// average, or every row when all_elem), 0 = tile pairs.
+__global__ void __launch_bounds__(256)
+k_step2_kind(const int *__restrict__ Aptr, const int *__restrict__ Annz, int tilem, int all_elem,
+             unsigned char *__restrict__ kind) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= tilem) return;
+    const int a0 = Aptr[I], a1 = Aptr[I + 1];
+    const long long e = (long long)Annz[a1] - Annz[a0];
+    kind[I] = (unsigned char)((all_elem || e <= 4LL * (a1 - a0)) ? 1 : 0);
 }
 
 // Upper bound on the segments of a C with tilem tile-rows and numtile
@@ -294,6 +309,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     if ((int)blockIdx.x >= ns) return;  // block-uniform
     const int w = (int)tsg_xcd_block(blockIdx.x, (unsigned)ns);
     const int4 sg = segs[w];
+    if (sg.w) return;  // element-wise segment (k_step2_elem)
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     const bool whole = sg.y == 0 && sg.z == Cptr[I + 1] - Cptr[I];
@@ -387,6 +403,101 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             pt[q4] = o;
         }
         const long long t = c0 + s;
+        uint4 *cm4 = reinterpret_cast<uint4 *>(Cmask + t * 16);
+        cm4[0] = make_uint4(mk[0], mk[1], mk[2], mk[3]);
+        cm4[1] = make_uint4(mk[4], mk[5], mk[6], mk[7]);
+        *reinterpret_cast<uint4 *>(Cptr16 + t * 16) = make_uint4(pt[0], pt[1], pt[2], pt[3]);
+        Cnnz[t] = sum;
+    }
+}
+
+// Element-wise bitmask symbolic for hypersparse tile-rows (B carries its
+// row index): one 256-thread block per segment; each thread takes whole A
+// tiles of the tile-row and, for each of their entries (r, k), ORs the
+// columns of B row k that fall in the segment's range into the row-r mask
+// of their C tile (slot found through an LDS hash of the segment's tile
+// columns).  Work is one step per intermediate product instead of one per
+// (A tile, B tile) pair; OR is order-free, so LDS atomics are exact.
+#define G2E_HASH 1024  // >= 2 * G2_UMAX
+
+__device__ __forceinline__ unsigned col_hash10(int J) { return ((unsigned)J * 2654435761u) >> 22; }
+
+__global__ void __launch_bounds__(G2_THREADS)
+k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
+             const int4 *__restrict__ segs, const int *__restrict__ nseg, const int *__restrict__ Cptr,
+             const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
+             int *__restrict__ Cnnz) {
+    __shared__ unsigned s_m[G2_UMAX * 8];  // two 16-bit rows per word, even row high
+    __shared__ int s_hkey[G2E_HASH];
+    __shared__ unsigned short s_hval[G2E_HASH];
+    const int tid = threadIdx.x;
+    const int ns = *nseg;
+    if ((int)blockIdx.x >= ns) return;  // block-uniform
+    const int w = (int)tsg_xcd_block(blockIdx.x, (unsigned)ns);
+    const int4 sg = segs[w];
+    if (!sg.w) return;  // tile-pair segment (k_step2_rows)
+    const int I = sg.x;
+    const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
+    for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
+    for (int i = tid; i < G2E_HASH; i += G2_THREADS) s_hkey[i] = -1;
+    __syncthreads();
+    for (int i = tid; i < u; i += G2_THREADS) {
+        const int J = Ccol[c0 + i];
+        unsigned h = col_hash10(J);
+        while (atomicCAS(&s_hkey[h], -1, J) != -1) h = (h + 1) & (G2E_HASH - 1);  // u <= G2E_HASH / 2
+        s_hval[h] = (unsigned short)i;
+    }
+    __syncthreads();
+    const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
+    const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
+    for (int t = a0 + tid; t < a1; t += G2_THREADS) {
+        const int k16 = op.Acol[t] * 16;
+        const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)t * 16);
+        const uint4 aA = am4[0], aB = am4[1];
+        const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
+        for (int r = 0; r < 16; r++) {
+            unsigned am = (aw[r >> 1] >> ((r & 1) * 16)) & 0xffffu;
+            while (am) {
+                const int b = 31 - __clz(am);
+                am &= ~(1u << b);
+                const int k = k16 + 15 - b;
+                const int e1 = Browp[k + 1];
+                for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); e < e1; e++) {
+                    const int col = Bcolx[e];
+                    if (col >= col_hi) break;
+                    const int J = col >> 4;
+                    unsigned h = col_hash10(J);  // J is in the segment's tile list
+                    int probe = 0;
+                    while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
+                    if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
+                    const unsigned bit = 1u << (15 - (col & 15));
+                    atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int sl = tid; sl < u; sl += G2_THREADS) {
+        unsigned w8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) w8[k] = s_m[sl * 8 + k];
+        unsigned mk[8], pt[4];
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) mk[k] = (w8[k] >> 16) | ((w8[k] & 0xffffu) << 16);
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+            unsigned o = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int r = q4 * 4 + b;
+                const unsigned row = (r & 1) ? (w8[r >> 1] & 0xffffu) : (w8[r >> 1] >> 16);
+                o |= (unsigned)sum << (8 * b);
+                sum += __popc(row);
+            }
+            pt[q4] = o;
+        }
+        const long long t = c0 + sl;
         uint4 *cm4 = reinterpret_cast<uint4 *>(Cmask + t * 16);
         cm4[0] = make_uint4(mk[0], mk[1], mk[2], mk[3]);
         cm4[1] = make_uint4(mk[4], mk[5], mk[6], mk[7]);
@@ -668,45 +779,56 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int r = tid >> 4, l = tid & 15;
     const int row = 16 * I + r;
     const int gb = tid & 48;  // first lane of this group in its wave
+    __shared__ int s_gex[16][16];             // per group: entries before each lane's tile
+    __shared__ unsigned short s_gam[16][16];  // per group: each lane's tile row mask
     if (row < m && nseg_nnz > 0) {
         const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-        // the 16 lanes load 16 A tiles' row masks at once; the tiles with an
-        // entry in row r are then taken one by one in ascending K
         for (int t0 = a0; t0 < a1; t0 += 16) {
+            // 16 A tiles per round, one per lane: their row-r entries are
+            // numbered x = 0..total-1 in ascending k (tiles in ascending K,
+            // columns ascending inside a tile)
             const int tl = t0 + l;
             const unsigned am_l = tl < a1 ? op.Amask[(long long)tl * 16 + r] : 0u;
-            unsigned nz = (unsigned)(__ballot(am_l != 0) >> gb) & 0xffffu;
-            while (nz) {
-                const int src = __ffs(nz) - 1;
-                nz &= nz - 1;
-                const int t = t0 + src;
-                unsigned am = (unsigned)__shfl((int)am_l, gb + src, 64);
-                const double *av = op.Aval + op.Annz[t] + op.Aptr16[(long long)t * 16 + r];
-                const int k16 = op.Acol[t] * 16;
-                int ia = 0;
-                while (am) {
+            const int cnt = __popc(am_l);
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const int v = __shfl_up(incl, o, 16);
+                if (l >= o) incl += v;
+            }
+            const int total = __shfl(incl, 15, 16);
+            s_gex[r][l] = incl - cnt;
+            s_gam[r][l] = (unsigned short)am_l;
+            tsg_wave_sync();
+            for (int x0 = 0; x0 < total; x0 += 16) {
+                // lane l takes entry x0 + l and finds its B range in parallel
+                const int x = x0 + l;
+                int k = 0, e0 = 0, e1 = 0;
+                double a = 0.0;
+                bool have = false;
+                if (x < total) {
+                    int ow = 0;
+#pragma unroll
+                    for (int st = 8; st > 0; st >>= 1)
+                        if (ow + st < 16 && s_gex[r][ow + st] <= x) ow += st;
+                    unsigned am = s_gam[r][ow];
+                    int j = x - s_gex[r][ow];  // j-th entry of the owner's row
+                    for (int q = 0; q < j; q++) am &= ~(1u << (31 - __clz(am)));
                     const int b = 31 - __clz(am);
-                    am &= ~(1u << b);
-                    const int k = k16 + 15 - b;
-                    const double a = av[ia++];
-                    const int e1 = Browp[k + 1];
-                    // first entry of B row k at or after col_lo: 16-ary search,
-                    // one probe per lane per round
-                    int lo = Browp[k], hi = e1;  // answer in [lo, hi]
-                    while (hi - lo > 1) {
-                        const int step = (hi - lo + 15) >> 4;
-                        const int pos = lo + l * step;
-                        const bool ge = pos < hi && Bcolx[pos] >= col_lo;
-                        const unsigned gm = (unsigned)(__ballot(ge) >> gb) & 0xffffu;
-                        const int f = gm ? __ffs(gm) - 1 : 16;  // first probe at or past col_lo
-                        if (f == 0) { hi = lo; break; }
-                        // the answer follows the last probe before col_lo
-                        const int last = f < 16 ? f - 1 : min(15, (hi - lo - 1) / step);
-                        if (f < 16) hi = lo + f * step;
-                        lo = lo + last * step + 1;
-                    }
-                    const int e0 = (hi > lo && Bcolx[lo] >= col_lo) ? lo : hi;
-                    for (int e = e0 + l; e < e1; e += 16) {
+                    const int t = t0 + ow;
+                    k = op.Acol[t] * 16 + 15 - b;
+                    a = op.Aval[op.Annz[t] + op.Aptr16[(long long)t * 16 + r] + j];
+                    e1 = Browp[k + 1];
+                    e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
+                    have = e0 < e1 && Bcolx[e0] < col_hi;
+                }
+                unsigned hv = (unsigned)(__ballot(have) >> gb) & 0xffffu;
+                while (hv) {  // entries with B entries in range, ascending k
+                    const int src = __ffs(hv) - 1;
+                    hv &= hv - 1;
+                    const double as = __shfl(a, gb + src, 64);
+                    const int es0 = __shfl(e0, gb + src, 64), es1 = __shfl(e1, gb + src, 64);
+                    for (int e = es0 + l; e < es1; e += 16) {
                         const int col = Bcolx[e];
                         if (col >= col_hi) break;
                         const int J = col >> 4, c = col & 15;
@@ -717,11 +839,12 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                         const int lo2 = s_hval[h];
                         const unsigned cm = s_cmask[lo2 * 16 + r];
                         double *acc = &s_acc[s_coff[lo2] + s_cptr[lo2 * 16 + r] + __popc(cm >> (16 - c))];
-                        *acc = __builtin_fma(a, Bvalx[e], *acc);
+                        *acc = __builtin_fma(as, Bvalx[e], *acc);
                     }
                     tsg_wave_sync();  // the next k of this row reads what lanes wrote
                 }
             }
+            tsg_wave_sync();  // s_gex / s_gam are rewritten next round
         }
     }
     __syncthreads();
@@ -1201,20 +1324,20 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
 // w.segcnt[tilem].
 template <int MODE>
 static int launch_segments(hipStream_t s, const tsg_dtiles *C, const SpgemmWork &w, int want, int max_tiles,
-                           int max_nnz, int4 *items) {
+                           int max_nnz, int4 *items, const unsigned char *cls) {
     const unsigned g = tsg_div_up(C->tilem, 256);
-    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem,
-                       (const unsigned char *)w.rowcls, want, max_tiles, max_nnz, w.segcnt, (int4 *)nullptr);
+    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem, cls, want,
+                       max_tiles, max_nnz, w.segcnt, (int4 *)nullptr);
     TSG_LAUNCH_CHECK();
     TSG_TRY(tsg_scan_i32_enqueue(s, w.segcnt, w.segcnt, C->tilem, w.part3));
-    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem,
-                       (const unsigned char *)w.rowcls, want, max_tiles, max_nnz, w.segcnt, items);
+    hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem, cls, want,
+                       max_tiles, max_nnz, w.segcnt, items);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
 
-static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, const tsg_spgemm_plan &P,
-                        const SpgemmWork &w) {
+static int launch_step2(hipStream_t s, const TileOperands &op, const tsg_dtiles *B, tsg_dtiles *C,
+                        const tsg_spgemm_plan &P, const SpgemmWork &w) {
     if (C->numtile <= 0) return TSG_OK;
     if (P.all) {
         const unsigned nblk = tp_grid(tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE));
@@ -1223,11 +1346,22 @@ static int launch_step2(hipStream_t s, const TileOperands &op, tsg_dtiles *C, co
         TSG_LAUNCH_CHECK();
         return TSG_OK;
     }
-    TSG_TRY(launch_segments<0>(s, C, w, -1, G2_UMAX, 0, w.seg2));
+    if (P.s3_elem) {  // B has its row index: element-wise symbolic for hypersparse tile-rows
+        hipLaunchKernelGGL(k_step2_kind, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, op.Aptr, op.Annz,
+                           C->tilem, P.elem_all, w.rowcls);
+        TSG_LAUNCH_CHECK();
+    }
+    TSG_TRY(launch_segments<0>(s, C, w, -1, G2_UMAX, 0, w.seg2, P.s3_elem ? w.rowcls : nullptr));
     hipLaunchKernelGGL(k_step2_rows, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), 0, s, op, (const int4 *)w.seg2,
                        (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
                        C->tile_nnz);
     TSG_LAUNCH_CHECK();
+    if (P.s3_elem) {
+        hipLaunchKernelGGL(k_step2_elem, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), 0, s, op, (const int *)B->rowx_ptr,
+                           (const int *)B->rowx_col, (const int4 *)w.seg2, (const int *)(w.segcnt + C->tilem),
+                           C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr, C->tile_nnz);
+        TSG_LAUNCH_CHECK();
+    }
     return TSG_OK;
 }
 
@@ -1256,7 +1390,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
         if (ev) TSG_HIP(hipEventRecord(ev[1], s));
     }
     if (!P.all && P.s3_unfit && P.s3_elem) {
-        TSG_TRY(launch_segments<1>(s, C, w, 0, G3E_UMAX, G3E_NNZ, w.seg3));
+        TSG_TRY(launch_segments<1>(s, C, w, 0, G3E_UMAX, G3E_NNZ, w.seg3, w.rowcls));
         hipLaunchKernelGGL(k_step3_elem, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
@@ -1335,7 +1469,7 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
     // ---- step 2
     const TileOperands op = make_operands(A, B);
-    TSG_TRY(launch_step2(s, op, C, P, w));
+    TSG_TRY(launch_step2(s, op, B, C, P, w));
     if (timed && numtileC > 0) {
         hipLaunchKernelGGL(k_bins, dim3(std::min<unsigned>(2048, tsg_div_up(numtileC, 256))), dim3(256), 0, s,
                            C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
@@ -1410,7 +1544,7 @@ int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, 
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, w.part1));
     TSG_TRY(launch_step1(s, A, B, C, true));
     const TileOperands op = make_operands(A, B);
-    TSG_TRY(launch_step2(s, op, C, P, w));
+    TSG_TRY(launch_step2(s, op, B, C, P, w));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, w.part2));
     if (C->nnz > 0) {
         if (!P.all) TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
