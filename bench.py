@@ -95,6 +95,7 @@ def batch_leg(ts, ctx, count=512, seed=2, nmin=1000, nmax=65536, steps=5, autotu
             ctx.build_rowindex(ta)
             seq.append((ta, ta, None))
         else:
+            ctx.build_rowindex(ta)
             dt = ctx.transpose(d)
             tb = ctx.csr2tile(dt)
             ctx.build_csc(tb)
@@ -278,6 +279,7 @@ def rmat_leg(ts, scale=20, chunk_products=1500000000):
         dp = ctx.permute(d, old, new)
         ta = ctx.csr2tile(dp)
         ctx.build_csc(ta)
+        ctx.build_rowindex(ta)
         dpt = ctx.transpose(dp)
         tb = ctx.csr2tile(dpt)
         ctx.build_csc(tb)

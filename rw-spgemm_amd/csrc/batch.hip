@@ -231,7 +231,8 @@ static int build_group(tsg_batch *b, const tsg_dcsr *probs, int first, int cnt, 
         if (!st) st = tsg_tiles_build_rowindex(ctx, &g.A);
         g.B = g.A;
     } else if (!st) {
-        st = tsg_csr2tile(ctx, &pb, &g.B);
+        st = tsg_tiles_build_rowindex(ctx, &g.A);  // the element kernels walk A's rows
+        if (!st) st = tsg_csr2tile(ctx, &pb, &g.B);
         if (!st) st = tsg_tiles_build_csc(ctx, &g.B);
         if (!st) st = tsg_tiles_build_rowindex(ctx, &g.B);
         tsg_dcsr_free(ctx, &pb);

@@ -136,6 +136,9 @@ struct TileOperands {
     const unsigned char *Bptr16;
     const unsigned short *Bmask;
     const double *Bval;
+    // A's element row index (may be NULL): rows' entries in ascending k
+    const int *Arowp, *Acolx;
+    const double *Avalx;
 };
 
 // Index p in [0, len) with list[p] == key, or -1 (list ascending).
@@ -450,31 +453,58 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     __syncthreads();
     const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-    for (int t = a0 + tid; t < a1; t += G2_THREADS) {
-        const int k16 = op.Acol[t] * 16;
-        const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)t * 16);
-        const uint4 aA = am4[0], aB = am4[1];
-        const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
-        for (int r = 0; r < 16; r++) {
-            unsigned am = (aw[r >> 1] >> ((r & 1) * 16)) & 0xffffu;
-            while (am) {
-                const int b = 31 - __clz(am);
-                am &= ~(1u << b);
-                const int k = k16 + 15 - b;
-                const int e1 = Browp[k + 1];
-                for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); e < e1; e++) {
-                    const int col = Bcolx[e];
-                    if (col >= col_hi) break;
-                    const int J = col >> 4;
-                    unsigned h = col_hash10(J);  // J is in the segment's tile list
-                    int probe = 0;
-                    while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
-                    if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
-                    const unsigned bit = 1u << (15 - (col & 15));
-                    atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
-                }
+    __shared__ int s_tex[G2_THREADS + 1];  // entries before each tile of the chunk
+    for (int tc = a0; tc < a1; tc += G2_THREADS) {
+        // one chunk of A tiles: their entries numbered block-wide, one
+        // entry per thread per step (balanced whatever the tile densities)
+        const int t = tc + tid;
+        int cnt = 0;
+        if (t < a1) {
+            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)t * 16);
+            const uint4 aA = am4[0], aB = am4[1];
+            cnt = __popc(aA.x) + __popc(aA.y) + __popc(aA.z) + __popc(aA.w) + __popc(aB.x) + __popc(aB.y) +
+                  __popc(aB.z) + __popc(aB.w);
+        }
+        __shared__ int s_w[G2_THREADS / 64];
+        const int incl = tsg_wave_incl_scan(cnt);
+        if ((tid & 63) == 63) s_w[tid >> 6] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int q = 0; q < (tid >> 6); q++) woff += s_w[q];
+        s_tex[tid] = woff + incl - cnt;
+        if (tid == G2_THREADS - 1) s_tex[G2_THREADS] = woff + incl;
+        __syncthreads();
+        const int total = s_tex[G2_THREADS];
+        for (int x = tid; x < total; x += G2_THREADS) {
+            int ow = 0;  // tile of entry x: last with s_tex <= x
+#pragma unroll
+            for (int st = G2_THREADS / 2; st > 0; st >>= 1)
+                if (ow + st < G2_THREADS && s_tex[ow + st] <= x) ow += st;
+            const int tt = tc + ow;
+            int j = x - s_tex[ow];
+            const unsigned short *am16 = op.Amask + (long long)tt * 16;
+            int r = 0;
+            unsigned am = am16[0];
+            while (r < 15 && j >= __popc(am)) {  // the row holding the j-th entry of the tile
+                j -= __popc(am);
+                am = am16[++r];
+            }
+            for (int q = 0; q < j; q++) am &= ~(1u << (31 - __clz(am)));
+            const int k = op.Acol[tt] * 16 + 15 - (31 - __clz(am));
+            const int e1 = Browp[k + 1];
+            for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); e < e1; e++) {
+                const int col = Bcolx[e];
+                if (col >= col_hi) break;
+                const int J = col >> 4;
+                unsigned h = col_hash10(J);  // J is in the segment's tile list
+                int probe = 0;
+                while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
+                if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
+                const unsigned bit = 1u << (15 - (col & 15));
+                atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
             }
         }
+        __syncthreads();  // s_tex is rewritten by the next chunk
     }
     __syncthreads();
     for (int sl = tid; sl < u; sl += G2_THREADS) {
@@ -723,18 +753,20 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // Segments of tile-rows the row-wise tile kernel cannot hold (too many C
 // tiles, A tiles or B tiles, typically hypersparse rows where most tiles
 // carry one or two nonzeros): element-level Gustavson with B's row index.
-// One 256-thread block per segment (<= G3E_UMAX C tiles, <= G3E_NNZ C
-// nonzeros, accumulated in LDS in final order); 16 groups of 16 lanes,
-// group r owns C row 16I+r.  The group walks A row 16I+r in ascending k
-// (A tiles in ascending K, bits in ascending column) and, for each a_ik,
-// its lanes scatter the entries of B row k that fall in the segment's
-// column range: distinct C entries per step, steps in ascending k, so each
-// C entry again sums its products in the serial SPA order, without atomics.
-#define G3E_UMAX 256
+// One 256-thread block per segment (a run of C tiles holding <= G3E_NNZ
+// nonzeros, accumulated in LDS in final order and located through an LDS
+// hash of the segment's C entries); 16 groups of 16 lanes, group r owns C
+// row 16I+r.  The group walks A row 16I+r in ascending k (A tiles in
+// ascending K, bits in ascending column) and, for each a_ik, its lanes
+// scatter the entries of B row k that fall in the segment's column range:
+// distinct C entries per step, steps in ascending k, so each C entry again
+// sums its products in the serial SPA order, without atomics.
 #define G3E_NNZ 2048
-#define G3E_HASH 512  // open-addressing table: C tile column -> slot (load <= 1/2)
 
-__device__ __forceinline__ unsigned col_hash(int J) { return ((unsigned)J * 2654435761u) >> 23; }  // 9 bits
+// LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
+// stored + 1 (0 = empty), value = the entry's position in the segment.
+#define G3E_EHASH 4096  // >= 2 * G3E_NNZ
+__device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 2654435761u) >> 20; }  // 12 bits
 
 __global__ void __launch_bounds__(256)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
@@ -742,13 +774,9 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
              double *__restrict__ Cval) {
-    __shared__ int s_ccol[G3E_UMAX];
-    __shared__ int s_coff[G3E_UMAX];
-    __shared__ unsigned short s_cmask[G3E_UMAX * 16];
-    __shared__ unsigned char s_cptr[G3E_UMAX * 16];
     __shared__ double s_acc[G3E_NNZ];
-    __shared__ int s_hkey[G3E_HASH];
-    __shared__ unsigned char s_hval[G3E_HASH];
+    __shared__ unsigned s_ekey[G3E_EHASH];
+    __shared__ unsigned short s_epos[G3E_EHASH];
     const int tid = threadIdx.x;
     // the grid is an upper bound: the first *nseg blocks (spread evenly over
     // the XCDs) take the segments, each XCD a contiguous range
@@ -759,29 +787,75 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     const int nbase = Cnnz[c0], nseg_nnz = Cnnz[c0 + u] - nbase;
+    const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
     for (int i = tid; i < nseg_nnz; i += 256) s_acc[i] = 0.0;
-    for (int i = tid; i < G3E_HASH; i += 256) s_hkey[i] = -1;
+    for (int i = tid; i < G3E_EHASH; i += 256) s_ekey[i] = 0u;
     __syncthreads();
+    // every C entry of the segment into the hash (one C tile per thread;
+    // structurally empty tiles skipped)
     for (int i = tid; i < u; i += 256) {
-        const int J = Ccol[c0 + i];
-        unsigned h = col_hash(J);
-        while (atomicCAS(&s_hkey[h], -1, J) != -1) h = (h + 1) & (G3E_HASH - 1);  // u <= G3E_HASH / 2
-        s_hval[h] = (unsigned char)i;
-        s_ccol[i] = J;
-        s_coff[i] = Cnnz[c0 + i] - nbase;
-        const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
-        reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
-        reinterpret_cast<uint4 *>(s_cmask)[2 * i + 1] = cm[1];
-        reinterpret_cast<uint4 *>(s_cptr)[i] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + i) * 16)[0];
+        const long long t = c0 + i;
+        int pos = Cnnz[t] - nbase;
+        if (Cnnz[t + 1] - Cnnz[t] == 0) continue;
+        const uint4 *cm4 = reinterpret_cast<const uint4 *>(Cmask + t * 16);
+        const uint4 mA = cm4[0], mB = cm4[1];
+        const unsigned mw[8] = {mA.x, mA.y, mA.z, mA.w, mB.x, mB.y, mB.z, mB.w};
+        const unsigned j16 = (unsigned)(Ccol[t] * 16 - col_lo);
+        for (int rr = 0; rr < 16; rr++) {  // rows in order: positions follow
+            unsigned cm = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
+            while (cm) {
+                const int bb = 31 - __clz(cm);
+                cm &= ~(1u << bb);
+                const unsigned key = (((j16 + 15 - bb) << 4) | (unsigned)rr) + 1u;
+                unsigned h = ent_hash(key);
+                while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
+                s_epos[h] = (unsigned short)pos++;
+            }
+        }
     }
     __syncthreads();
-    const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
     const int r = tid >> 4, l = tid & 15;
     const int row = 16 * I + r;
     const int gb = tid & 48;  // first lane of this group in its wave
     __shared__ int s_gex[16][16];             // per group: entries before each lane's tile
     __shared__ unsigned short s_gam[16][16];  // per group: each lane's tile row mask
-    if (row < m && nseg_nnz > 0) {
+    if (row < m && nseg_nnz > 0 && op.Arowp) {
+        // A's row index: row 16I+r's entries are contiguous, ascending k; the
+        // 16 lanes take 16 of them and find their B ranges in parallel
+        const int x1 = op.Arowp[row + 1];
+        for (int x0 = op.Arowp[row]; x0 < x1; x0 += 16) {
+            const int x = x0 + l;
+            int e0 = 0, e1 = 0;
+            double a = 0.0;
+            bool have = false;
+            if (x < x1) {
+                const int k = op.Acolx[x];
+                a = op.Avalx[x];
+                e1 = Browp[k + 1];
+                e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
+                have = e0 < e1 && Bcolx[e0] < col_hi;
+            }
+            unsigned hv = (unsigned)(__ballot(have) >> gb) & 0xffffu;
+            while (hv) {  // entries with B entries in range, ascending k
+                const int src = __ffs(hv) - 1;
+                hv &= hv - 1;
+                const double as = __shfl(a, gb + src, 64);
+                const int es0 = __shfl(e0, gb + src, 64), es1 = __shfl(e1, gb + src, 64);
+                for (int e = es0 + l; e < es1; e += 16) {
+                    const int col = Bcolx[e];
+                    if (col >= col_hi) break;
+                    const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
+                    unsigned h = ent_hash(key);  // the entry is in C's structure
+                    int probe = 0;
+                    while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
+                    if (probe == G3E_EHASH) continue;  // not reached for a consistent C structure
+                    double *acc = &s_acc[s_epos[h]];
+                    *acc = __builtin_fma(as, Bvalx[e], *acc);
+                }
+                tsg_wave_sync();  // the next k of this row reads what lanes wrote
+            }
+        }
+    } else if (row < m && nseg_nnz > 0) {
         const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
         for (int t0 = a0; t0 < a1; t0 += 16) {
             // 16 A tiles per round, one per lane: their row-r entries are
@@ -831,14 +905,12 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     for (int e = es0 + l; e < es1; e += 16) {
                         const int col = Bcolx[e];
                         if (col >= col_hi) break;
-                        const int J = col >> 4, c = col & 15;
-                        unsigned h = col_hash(J);  // J is in the segment's tile list
+                        const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
+                        unsigned h = ent_hash(key);  // the entry is in C's structure
                         int probe = 0;
-                        while (s_hkey[h] != J && ++probe < G3E_HASH) h = (h + 1) & (G3E_HASH - 1);
-                        if (probe == G3E_HASH) continue;  // not reached for a consistent C structure
-                        const int lo2 = s_hval[h];
-                        const unsigned cm = s_cmask[lo2 * 16 + r];
-                        double *acc = &s_acc[s_coff[lo2] + s_cptr[lo2 * 16 + r] + __popc(cm >> (16 - c))];
+                        while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
+                        if (probe == G3E_EHASH) continue;  // not reached for a consistent C structure
+                        double *acc = &s_acc[s_epos[h]];
                         *acc = __builtin_fma(as, Bvalx[e], *acc);
                     }
                     tsg_wave_sync();  // the next k of this row reads what lanes wrote
@@ -849,15 +921,20 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     }
     __syncthreads();
     for (int i = tid; i < nseg_nnz; i += 256) Cval[(long long)nbase + i] = s_acc[i];
-    for (int x = tid; x < u * 16; x += 256) {
-        const int slot = x >> 4;
-        unsigned cm = s_cmask[x];
-        unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
-        int jj = 0;
-        while (cm) {
-            const int bb = 31 - __clz(cm);
-            cm &= ~(1u << bb);
-            oc[jj++] = (unsigned char)(15 - bb);
+    for (int i = tid; i < u; i += 256) {
+        const long long t = c0 + i;
+        if (Cnnz[t + 1] - Cnnz[t] == 0) continue;
+        const uint4 *cm4 = reinterpret_cast<const uint4 *>(Cmask + t * 16);
+        const uint4 mA = cm4[0], mB = cm4[1];
+        const unsigned mw[8] = {mA.x, mA.y, mA.z, mA.w, mB.x, mB.y, mB.z, mB.w};
+        unsigned char *oc = Ccol8 + Cnnz[t];
+        for (int rr = 0; rr < 16; rr++) {
+            unsigned cm = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
+            while (cm) {
+                const int bb = 31 - __clz(cm);
+                cm &= ~(1u << bb);
+                *oc++ = (unsigned char)(15 - bb);
+            }
         }
     }
 }
@@ -1252,6 +1329,7 @@ static TileOperands make_operands(const tsg_dtiles *A, const tsg_dtiles *B) {
     op.Bptr = B->tile_ptr; op.Bcol = B->tile_colidx;
     op.Bcscptr = B->csc_ptr; op.Bcscrow = B->csc_rowidx; op.Bcscid = B->csc_tileid; op.Bnnz = B->tile_nnz;
     op.Bptr16 = B->csr_ptr; op.Bmask = B->mask; op.Bval = B->val;
+    op.Arowp = A->rowx_ptr; op.Acolx = A->rowx_col; op.Avalx = A->rowx_val;
     return op;
 }
 
@@ -1293,7 +1371,7 @@ static size_t work_layout(long long tilem, long long numtile, long long nnz, siz
         8 * (size_t)tsg_scan_partial_count(tilem), 8 * (size_t)tsg_scan_partial_count(numtile),
         8 * (size_t)tsg_scan_partial_count(tilem), 4 * ((size_t)tilem + 1), (size_t)tilem + 1, 32,
         16 * (size_t)seg_bound(tilem, numtile, G2_UMAX, 0, 0),
-        16 * (size_t)(tilem + numtile / G3E_UMAX + nnz / (G3E_NNZ - 255) + 2)};
+        16 * (size_t)(tilem + nnz / (G3E_NNZ - 255) + 2)};
     size_t o = 0;
     for (int i = 0; i < 8; i++) {
         off[i] = o;
@@ -1316,7 +1394,7 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.seg2 = (int4 *)(b + off[6]);
     w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[7]);
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
-    w.seg3_cap = tilem + numtile / G3E_UMAX + nnz / (G3E_NNZ - 255) + 2;
+    w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
     return w;
 }
 
@@ -1390,7 +1468,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
         if (ev) TSG_HIP(hipEventRecord(ev[1], s));
     }
     if (!P.all && P.s3_unfit && P.s3_elem) {
-        TSG_TRY(launch_segments<1>(s, C, w, 0, G3E_UMAX, G3E_NNZ, w.seg3, w.rowcls));
+        TSG_TRY(launch_segments<1>(s, C, w, 0, 0x7fffffff, G3E_NNZ, w.seg3, w.rowcls));
         hipLaunchKernelGGL(k_step3_elem, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
@@ -1430,7 +1508,9 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     memset(&P, 0, sizeof(P));
     P.all = tune.force_tile_parallel ? 1 : 0;
     P.dense_min = tune.dense_acc_min_nnz;
-    P.s3_elem = B->rowx_ptr != nullptr && tune.element_mode != 1;
+    // element path: B carries its row index, and C's columns fit the
+    // 28-bit entry keys of the element kernel's LDS hash
+    P.s3_elem = B->rowx_ptr != nullptr && tune.element_mode != 1 && B->n < (1 << 27);
     P.elem_all = B->rowx_ptr != nullptr && tune.element_mode == 2;
     hipStream_t s = ctx->stream;
 
@@ -1661,6 +1741,7 @@ extern "C" int tsg_spgemm_tile_rows(tsg_ctx *ctx, const tsg_dtiles *A, const tsg
     }
     tsg_dtiles v = *A;
     v.tile_ptr = A->tile_ptr + r0;
+    if (A->rowx_ptr) v.rowx_ptr = A->rowx_ptr + 16 * (long long)r0;  // indices stay absolute
     v.tilem = r1 - r0;
     v.m = std::min(16 * r1, A->m) - 16 * r0;
     if (v.m < 0) v.m = 0;
