@@ -85,13 +85,26 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
             pre[lane] = incl - len;
             pb0[lane] = b0;
             tsg_wave_sync();
-            for (int c = lane; c < tot; c += 64) {
-                int lo = 0;
+            if (tot >= 64 * 32) {
+                // long B tile-rows (32+ tiles on average): the 64 lanes sweep
+                // one B tile-row at a time, no candidate search
+                const int na = min(64, a1 - c0);
+                for (int j = 0; j < na; j++) {
+                    const int bj = pb0[j], lj = (j + 1 < na ? pre[j + 1] : tot) - pre[j];
+                    for (int c = lane; c < lj; c += 64) {
+                        const int d = Bcol[bj + c] - w0;
+                        if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                    }
+                }
+            } else {
+                for (int c = lane; c < tot; c += 64) {
+                    int lo = 0;
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1)
-                    if (pre[lo + st] <= c) lo += st;
-                const int d = Bcol[pb0[lo] + (c - pre[lo])] - w0;
-                if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                    for (int st = 32; st > 0; st >>= 1)
+                        if (pre[lo + st] <= c) lo += st;
+                    const int d = Bcol[pb0[lo] + (c - pre[lo])] - w0;
+                    if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                }
             }
             tsg_wave_sync();
         }
@@ -216,20 +229,30 @@ __device__ __forceinline__ int seg_walk(const int *__restrict__ Cnnz, int c0, in
             for (int j = 0; j < n; j++) out[j] = make_int4(I, j * max_tiles, min(u, (j + 1) * max_tiles), kind);
         return n;
     }
-    int n = 0, s0 = 0, acc = 0;
-    for (int s = 0; s < u; s++) {
-        const int tn = Cnnz[c0 + s + 1] - Cnnz[c0 + s];
-        if (s > s0 && (s - s0 == max_tiles || acc + tn > max_nnz)) {
-            if (out) out[n] = make_int4(I, s0, s, 0);
-            n++;
-            s0 = s;
-            acc = 0;
+    // MODE 1: tile s belongs to segment floor(p(s) / H), p(s) = C nonzeros of
+    // the row before tile s and H = max_nnz - 255; a tile holds at most 256
+    // nonzeros, so ids grow by at most one per tile, every id up to the last
+    // is present, and a segment holds at most H + 255 = max_nnz nonzeros.
+    // Count in O(1), each boundary by one binary search (no tile walk).
+    const int H = max_nnz - 255;
+    const int base = Cnnz[c0];
+    const int n = (Cnnz[c0 + u - 1] - base) / H + 1;
+    if (out) {
+        int s0 = 0;
+        for (int j = 0; j < n; j++) {
+            int s1 = u;
+            if (j + 1 < n) {  // first tile with p >= (j+1) H
+                const int key = base + (j + 1) * H;
+                int lo = s0, hi = u - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (Cnnz[c0 + mid] < key) lo = mid + 1; else hi = mid;
+                }
+                s1 = lo;
+            }
+            out[j] = make_int4(I, s0, s1, 0);
+            s0 = s1;
         }
-        acc += tn;
-    }
-    if (u > 0) {
-        if (out) out[n] = make_int4(I, s0, u, 0);
-        n++;
     }
     return n;
 }
