@@ -6,8 +6,9 @@ U[-1,1) from seed 1) on one MI355X.
 A step is one full device pipeline (step 1 tile symbolic, step 2 bitmask
 symbolic, step 3 numeric, including the pooled allocations of C) with A
 already resident in HBM; GFLOPS = 2 * nnzCub / t (src/tilespgemm-cuda.h:
-3185).  Prints one JSON line (see the keys below).  --gpus N > 1 exits: this
-build measures one GPU.
+3185).  Prints one JSON line (see the keys below), with the batched and
+served workloads under "batched" and "served".  --gpus N > 1 exits (one GPU
+only).
 """
 import argparse
 import json
@@ -364,8 +365,8 @@ def main():
                     help="first run the plain copy kernel over this many bytes (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(SpGEMM would shard by C tile-rows, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
+              "(a multi-GPU SpGEMM would shard by C tile-rows with B replicated)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import tilespgemm as ts

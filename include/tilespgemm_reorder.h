@@ -103,7 +103,7 @@ int build_external_reorder_plan(ReorderProblemKind kind, const char *algorithm, 
 int build_original_reorder_plan(ReorderProblemKind kind, const SMatrix *matrixA, const SMatrix *matrixB,
                                 ReorderInfo *info);
 
-/* The reference's TASA/GTSP heuristic planner is not part of this build;
+/* The reference's TASA/GTSP heuristic planner is not implemented here;
  * this entry point builds the RCM plan of build_external_reorder_plan for
  * the problem instead (mode "rcm").
  * Replaces: int build_reorder_plan(...)  src/reorder.h:2407 */
