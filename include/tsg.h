@@ -166,7 +166,12 @@ typedef struct {
                               kernel cannot hold, when B has a row index:
                               0 = element-wise (default), 1 = tile-parallel,
                               2 = element-wise for every tile-row */
-    int reserved[3];
+    int no_heavy_rows;     /* 1: rows with very many products stay in the
+                              element-wise segments instead of one wave each
+                              (default 0) */
+    int heavy_min_products; /* a row with more intermediate products (or more
+                              than 2048 A entries) is heavy (0 = 65536) */
+    int reserved[1];
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 
@@ -175,6 +180,8 @@ typedef struct {
     long long numtileC, nnzC;
     long long tiles_sparse_acc, tiles_dense_acc, tiles_empty;
     double t_step3_rows_ms; /* the row-wise numeric kernel's launch alone */
+    long long heavy_rows;   /* rows computed one wave each (k_step3_heavy) */
+    long long heavy_bytes;  /* transient device bytes of that path */
 } tsg_stats;
 
 /* tune and stats may be NULL.  When stats is given, each step is bracketed

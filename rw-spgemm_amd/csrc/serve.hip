@@ -167,6 +167,7 @@ static int size_product(tsg_server *s, tsg_ctx *ctx, const tsg_dtiles &A, const 
     bounds.resize(n + 1);
     os.bounds = bounds;
     os.numtileC = os.nnzC = os.bytes = 0;
+    long long transient = 0;
     for (int c = 0; c < n; c++) {
         tsg_dtiles C;
         tsg_stats st;
@@ -174,9 +175,10 @@ static int size_product(tsg_server *s, tsg_ctx *ctx, const tsg_dtiles &A, const 
         os.numtileC += st.numtileC;
         os.nnzC += st.nnzC;
         os.bytes += c_bytes(C.tilem, st.numtileC, st.nnzC);
+        transient = std::max(transient, st.heavy_bytes);  // freed after each chunk
         tsg_tiles_free(ctx, &C);
     }
-    os.bytes += os.bytes / 32 + (1 << 20);  // allocator rounding and small scratch
+    os.bytes += transient + os.bytes / 32 + (1 << 20);  // + allocator rounding and small scratch
     os.ok = true;
     return TSG_OK;
 }

@@ -796,7 +796,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
              const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
-             double *__restrict__ Cval) {
+             double *__restrict__ Cval, const unsigned char *__restrict__ heavy) {
     __shared__ double s_acc[G3E_NNZ];
     __shared__ unsigned s_ekey[G3E_EHASH];
     __shared__ unsigned short s_epos[G3E_EHASH];
@@ -839,10 +839,13 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     __syncthreads();
     const int r = tid >> 4, l = tid & 15;
     const int row = 16 * I + r;
+    // rows flagged heavy are computed by k_step3_heavy, which runs after
+    // this kernel and overwrites their values
+    const bool skip = heavy && row < m && heavy[row];
     const int gb = tid & 48;  // first lane of this group in its wave
     __shared__ int s_gex[16][16];             // per group: entries before each lane's tile
     __shared__ unsigned short s_gam[16][16];  // per group: each lane's tile row mask
-    if (row < m && nseg_nnz > 0 && op.Arowp) {
+    if (row < m && nseg_nnz > 0 && op.Arowp && !skip) {
         // A's row index: row 16I+r's entries are contiguous, ascending k; the
         // 16 lanes take 16 of them and find their B ranges in parallel
         const int x1 = op.Arowp[row + 1];
@@ -878,7 +881,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                 tsg_wave_sync();  // the next k of this row reads what lanes wrote
             }
         }
-    } else if (row < m && nseg_nnz > 0) {
+    } else if (row < m && nseg_nnz > 0 && !skip) {
         const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
         for (int t0 = a0; t0 < a1; t0 += 16) {
             // 16 A tiles per round, one per lane: their row-r entries are
@@ -959,6 +962,106 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                 *oc++ = (unsigned char)(15 - bb);
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------ step 3, heavy rows
+// Rows with very many intermediate products (hub rows of power-law
+// graphs) would make every C segment of their tile-row re-walk the whole
+// A row.  They get one wave each instead: the wave walks A row i once in
+// ascending k, its lanes accumulating B row k into a dense FP64 row in
+// global memory (one per wave, spanning the C tile-row's columns), then
+// scatters the row into C's final positions and clears what it touched.
+// Only lanes of one wave share a dense row, so workgroup-scope atomics and
+// fences order the k steps (no cache maintenance is needed within a CU).
+#define HEAVY_MIN_PRODUCTS 65536
+#define HEAVY_MAX_WAVES 1024
+
+__global__ void __launch_bounds__(256)
+k_heavy_flags(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const int *__restrict__ Browp,
+              const unsigned char *__restrict__ rowcls, const int *__restrict__ Cptr, const int *__restrict__ Ccol,
+              int m, long long min_products, unsigned char *__restrict__ flag, int *__restrict__ cnt,
+              int *__restrict__ span) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int h = 0;
+    if (i < m && rowcls[i >> 4] == 0) {
+        const int x0 = Arowp[i], x1 = Arowp[i + 1];
+        if (x1 - x0 > 2048) {
+            h = 1;
+        } else {
+            long long p = 0;
+            for (int x = x0; x < x1; x++) p += Browp[Acolx[x] + 1] - Browp[Acolx[x]];
+            h = p > min_products;
+        }
+        if (h) {
+            const int I = i >> 4, c0 = Cptr[I], c1 = Cptr[I + 1];
+            atomicMax(span, (Ccol[c1 - 1] - Ccol[c0] + 1) * 16);
+        }
+    }
+    if (i < m) {
+        flag[i] = (unsigned char)h;
+        cnt[i] = h;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_heavy_list(const int *__restrict__ off, const unsigned char *__restrict__ flag, int m, int *__restrict__ list) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m && flag[i]) list[off[i]] = i;
+}
+
+__device__ __forceinline__ double wg_load(double *p) {
+    const unsigned long long v =
+        __hip_atomic_load(reinterpret_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ void wg_store(double *p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wg_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__global__ void __launch_bounds__(256)
+k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const double *__restrict__ Avalx,
+              const int *__restrict__ Browp, const int *__restrict__ Bcolx, const double *__restrict__ Bvalx,
+              const int *__restrict__ hlist, const int *__restrict__ nheavy, const int *__restrict__ Cptr,
+              const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
+              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, double *__restrict__ Cval,
+              double *__restrict__ dense, long long span) {
+    const int lane = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+    double *row_acc = dense + (long long)wv * span;
+    const int nh = *nheavy;
+    for (int h = wv; h < nh; h += nw) {
+        const int i = hlist[h];
+        const int I = i >> 4, r = i & 15;
+        const int c0 = Cptr[I], c1 = Cptr[I + 1];
+        const int colbase = Ccol[c0] * 16;
+        for (int x = Arowp[i]; x < Arowp[i + 1]; x++) {  // ascending k
+            const int k = Acolx[x];
+            const double a = Avalx[x];
+            for (int e = Browp[k] + lane; e < Browp[k + 1]; e += 64) {  // distinct columns
+                double *d = row_acc + (Bcolx[e] - colbase);
+                wg_store(d, __builtin_fma(a, Bvalx[e], wg_load(d)));
+            }
+            wg_fence();  // the next k reads what this one wrote
+        }
+        for (int t = c0 + lane; t < c1; t += 64) {  // into C, clearing the dense row
+            unsigned cm = Cmask[(long long)t * 16 + r];
+            long long pos = (long long)Cnnz[t] + Cptr16[(long long)t * 16 + r];
+            double *d = row_acc + (Ccol[t] * 16 - colbase);
+            while (cm) {
+                const int bb = 31 - __clz(cm);
+                cm &= ~(1u << bb);
+                Cval[pos++] = wg_load(d + 15 - bb);
+                wg_store(d + 15 - bb, 0.0);
+            }
+        }
+        wg_fence();
     }
 }
 
@@ -1382,6 +1485,12 @@ struct SpgemmWork {
     int4 *seg2;        // step-2 segments
     int4 *seg3;        // step-3 element segments
     long long seg2_cap, seg3_cap;
+    // heavy rows (synchronous pipeline only; NULL = none)
+    unsigned char *hflag;  // per C row
+    int *hlist, *hcount;   // heavy rows, their number (device)
+    int nheavy, hwaves;
+    double *dense;         // hwaves dense rows of hspan doubles
+    long long hspan;
 };
 
 static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -1418,6 +1527,11 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[7]);
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
     w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
+    w.hflag = nullptr;
+    w.hlist = w.hcount = nullptr;
+    w.nheavy = w.hwaves = 0;
+    w.dense = nullptr;
+    w.hspan = 0;
     return w;
 }
 
@@ -1495,8 +1609,15 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
         hipLaunchKernelGGL(k_step3_elem, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
-                           C->tile_nnz, C->csr_col, C->val);
+                           C->tile_nnz, C->csr_col, C->val, (const unsigned char *)w.hflag);
         TSG_LAUNCH_CHECK();
+        if (w.hflag && w.nheavy > 0) {
+            hipLaunchKernelGGL(k_step3_heavy, dim3(tsg_div_up(w.hwaves, 4)), dim3(256), 0, s, op.Arowp, op.Acolx,
+                               op.Avalx, (const int *)B->rowx_ptr, (const int *)B->rowx_col,
+                               (const double *)B->rowx_val, (const int *)w.hlist, (const int *)w.hcount, C->tile_ptr,
+                               C->tile_colidx, C->mask, C->csr_ptr, C->tile_nnz, C->val, w.dense, w.hspan);
+            TSG_LAUNCH_CHECK();
+        }
     } else if (P.all || P.s3_unfit) {
         const unsigned nblk = tp_grid(tsg_div_up(C->numtile, S23_WAVES * TILES_PER_WAVE));
         hipLaunchKernelGGL(k_step3, dim3(nblk), dim3(64 * S23_WAVES), 0, s, op, C->tile_ptr, C->tile_rowidx,
@@ -1563,6 +1684,9 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     C->numtile = (int)numtileC;
     TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
     void *wbuf = nullptr, *w3buf = nullptr;
+    unsigned char *hflag = nullptr;
+    int *hcnt = nullptr, *hlist = nullptr, *hspan = nullptr;
+    double *hdense = nullptr;
     size_t woff[8];
     const size_t wbytes = work_layout(C->tilem, numtileC, 0, woff);
     TSG_TRY(tsg_alloc(ctx, &wbuf, woff[7]));  // everything but the step-3 segments
@@ -1602,6 +1726,40 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
             TSG_TRY(tsg_alloc(ctx, &w3buf, 16 * (size_t)w3.seg3_cap));
             w.seg3 = (int4 *)w3buf;
             w.seg3_cap = w3.seg3_cap;
+            if (op.Arowp && !tune.no_heavy_rows) {
+                // rows with very many products: one wave each (k_step3_heavy)
+                const int m = C->m;
+                TSG_TRY(tsg_alloc_t(ctx, &hflag, (size_t)m + 16));
+                TSG_TRY(tsg_alloc_t(ctx, &hcnt, (size_t)m + 1));
+                TSG_TRY(tsg_alloc_t(ctx, &hspan, 4));
+                TSG_HIP(hipMemsetAsync(hspan, 0, 4, s));
+                hipLaunchKernelGGL(k_heavy_flags, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, op.Arowp, op.Acolx,
+                                   (const int *)B->rowx_ptr, (const unsigned char *)w.rowcls, C->tile_ptr,
+                                   C->tile_colidx, m,
+                                   (long long)(tune.heavy_min_products > 0 ? tune.heavy_min_products
+                                                                           : HEAVY_MIN_PRODUCTS),
+                                   hflag, hcnt, hspan);
+                TSG_LAUNCH_CHECK();
+                long long nh = 0;
+                TSG_TRY(tsg_scan_i32(ctx, hcnt, hcnt, m, &nh));
+                if (nh > 0) {
+                    long long span = 0;
+                    TSG_TRY(tsg_read_scalar_i64(ctx, hspan, 0, &span));
+                    TSG_TRY(tsg_alloc_t(ctx, &hlist, (size_t)nh));
+                    hipLaunchKernelGGL(k_heavy_list, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, (const int *)hcnt,
+                                       (const unsigned char *)hflag, m, hlist);
+                    TSG_LAUNCH_CHECK();
+                    w.hwaves = (int)std::min<long long>(nh, HEAVY_MAX_WAVES);
+                    TSG_TRY(tsg_alloc_t(ctx, &hdense, (size_t)w.hwaves * (size_t)span));
+                    TSG_HIP(hipMemsetAsync(hdense, 0, (size_t)w.hwaves * (size_t)span * 8, s));
+                    w.hflag = hflag;
+                    w.hlist = hlist;
+                    w.hcount = hcnt + m;
+                    w.nheavy = (int)nh;
+                    w.dense = hdense;
+                    w.hspan = span;
+                }
+            }
         }
         TSG_TRY(launch_step3(s, op, B, C, P, w, timed ? ev + 4 : nullptr));
     }
@@ -1621,8 +1779,15 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         stats->tiles_dense_acc = (long long)hb[2];
         stats->numtileC = numtileC;
         stats->nnzC = nnzC;
+        stats->heavy_rows = w.nheavy;
+        stats->heavy_bytes = (long long)w.hwaves * w.hspan * 8 + (w.nheavy ? 9LL * C->m : 0);
         for (auto &e : ev) hipEventDestroy(e);
     }
+    tsg_free(ctx, hdense);
+    tsg_free(ctx, hlist);
+    tsg_free(ctx, hspan);
+    tsg_free(ctx, hcnt);
+    tsg_free(ctx, hflag);
     tsg_free(ctx, w3buf);
     tsg_free(ctx, wbuf);
     tsg_free(ctx, scr);

@@ -78,13 +78,15 @@ class HTiles(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("force_tile_parallel", _i),
-                ("batch_group_nnz_cap", _i), ("element_mode", _i), ("reserved", _i * 3)]
+                ("batch_group_nnz_cap", _i), ("element_mode", _i), ("no_heavy_rows", _i),
+                ("heavy_min_products", _i), ("reserved", _i * 1)]
 
 
 class Stats(C.Structure):
     _fields_ = [("t_step1_ms", _d), ("t_step2_ms", _d), ("t_step3_ms", _d), ("t_alloc_ms", _d),
                 ("t_total_ms", _d), ("numtileC", _ll), ("nnzC", _ll), ("tiles_sparse_acc", _ll),
-                ("tiles_dense_acc", _ll), ("tiles_empty", _ll), ("t_step3_rows_ms", _d)]
+                ("tiles_dense_acc", _ll), ("tiles_empty", _ll), ("t_step3_rows_ms", _d), ("heavy_rows", _ll),
+                ("heavy_bytes", _ll)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -79,13 +79,20 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     # element-wise numeric step with B's row index: for every tile-row, and
     # for the tile-rows the row-wise tile kernel cannot hold
     ctx.build_rowindex(tb)
-    for mode in (2, 0):
+    # with A's row index too, and rows above 64 products computed one wave
+    # each (the heavy-row path)
+    configs = [(2, 0), (0, 0)]
+    if b is not None:
+        ctx.build_rowindex(ta)
+    configs.append((2, 64))
+    for mode, heavy in configs:
         t3 = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
         t3.element_mode = mode
+        t3.heavy_min_products = heavy
         tc3, _ = ctx.spgemm(ta, tb, t3)
         C3 = ctx.download_tiles(tc3)
         for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
-            assert np.array_equal(C3[k], C[k]), f"element mode {mode} differs in {k}"
+            assert np.array_equal(C3[k], C[k]), f"element mode {mode} / heavy {heavy} differs in {k}"
         ctx.free(tc3)
     for obj in (dc, tc):
         ctx.free(obj)
