@@ -156,7 +156,8 @@ typedef struct {
                               LDS accumulator, smaller ones the compact
                               sparse one (default 64) */
     int s1_window_words;   /* step-1 LDS bitmap window, 32-bit words per
-                              wave (default 256) */
+                              wave: 256, 2048, or 0 = by B's width
+                              (default 0) */
     int force_tile_parallel; /* 1: steps 2 and 3 use the tile-parallel
                               kernels for every tile-row instead of the
                               row-wise LDS kernels (default 0 = by size) */

@@ -41,14 +41,14 @@ double clock_ms() {
 }
 
 // Device bytes of a C tile matrix plus the pipeline's work buffer.
-long long c_bytes(long long tilem, long long numtile, long long nnz) {
+long long c_bytes(long long tilem, long long numtile, long long nnz, int s1w) {
     tsg_dtiles c;
     memset(&c, 0, sizeof(c));
     c.tilem = (int)tilem;
     c.numtile = (int)numtile;
     c.nnz = (int)nnz;
     const long long arrays = 4 * (tilem + 1) + 12 * (numtile + 1) + 48 * numtile + 32 + 9 * (nnz + 1);
-    return arrays + (long long)tsg_spgemm_work_bytes(&c);
+    return arrays + (long long)tsg_spgemm_work_bytes(&c, s1w);
 }
 
 long long tile_bytes(const tsg_dtiles &t) {
@@ -174,7 +174,7 @@ static int size_product(tsg_server *s, tsg_ctx *ctx, const tsg_dtiles &A, const 
         TSG_TRY(tsg_spgemm_tile_rows(ctx, &A, &B, bounds[c], bounds[c + 1], &s->cfg.tune, &C, &st));
         os.numtileC += st.numtileC;
         os.nnzC += st.nnzC;
-        os.bytes += c_bytes(C.tilem, st.numtileC, st.nnzC);
+        os.bytes += c_bytes(C.tilem, st.numtileC, st.nnzC, tsg_s1_words(&B, s->cfg.tune));
         transient = std::max(transient, st.heavy_bytes);  // freed after each chunk
         tsg_tiles_free(ctx, &C);
     }

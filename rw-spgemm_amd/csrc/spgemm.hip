@@ -34,7 +34,7 @@
 extern "C" void tsg_tune_default(tsg_tune *t) {
     memset(t, 0, sizeof(*t));
     t->dense_acc_min_nnz = 64;
-    t->s1_window_words = 256;
+    t->s1_window_words = 0;
 }
 
 // ------------------------------------------------------------------ step 1
@@ -42,7 +42,7 @@ template <bool FILL, int BMW>
 __global__ void __launch_bounds__(64 * S1_WAVES)
 k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
         const int *__restrict__ Bptr, const int *__restrict__ Bcol,
-        int *__restrict__ Cptr_or_cnt, int *__restrict__ Ccol, int *__restrict__ Crow) {
+        int *__restrict__ Cptr_or_cnt, int *__restrict__ Ccol, int *__restrict__ Crow, unsigned *__restrict__ bmstore) {
     __shared__ unsigned s_bm[S1_WAVES][BMW];
     __shared__ int s_pre[S1_WAVES][64];
     __shared__ int s_b0[S1_WAVES][64];
@@ -68,45 +68,56 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
 
     int total = 0;
     const int base = FILL ? Cptr_or_cnt[I] : 0;
+    // a row whose candidates fit one window keeps the count pass's bitmap
+    // (bmstore, BMW words per tile-row) so the fill pass reads it back
+    const bool kept = bmstore != nullptr && jmax >= jmin && jmax - jmin < 32 * BMW;
+    unsigned *keep = kept ? bmstore + (long long)I * BMW : nullptr;
     for (int w0 = jmin; w0 <= jmax; w0 += 32 * BMW) {
         const int nw = min(BMW, ((jmax - w0) >> 5) + 1);
-        for (int k = lane; k < nw; k += 64) bm[k] = 0u;
-        tsg_wave_sync();
-        for (int c0 = a0; c0 < a1; c0 += 64) {
-            const int p = c0 + lane;
-            int b0 = 0, len = 0;
-            if (p < a1) {
-                const int K = Acol[p];
-                b0 = Bptr[K];
-                len = Bptr[K + 1] - b0;
-            }
-            const int incl = tsg_wave_incl_scan(len);
-            const int tot = __shfl(incl, 63, 64);
-            pre[lane] = incl - len;
-            pb0[lane] = b0;
+        if (FILL && kept) {
+            for (int k = lane; k < nw; k += 64) bm[k] = keep[k];
             tsg_wave_sync();
-            if (tot >= 64 * 32) {
-                // long B tile-rows (32+ tiles on average): the 64 lanes sweep
-                // one B tile-row at a time, no candidate search
-                const int na = min(64, a1 - c0);
-                for (int j = 0; j < na; j++) {
-                    const int bj = pb0[j], lj = (j + 1 < na ? pre[j + 1] : tot) - pre[j];
-                    for (int c = lane; c < lj; c += 64) {
-                        const int d = Bcol[bj + c] - w0;
+        } else {
+            for (int k = lane; k < nw; k += 64) bm[k] = 0u;
+            tsg_wave_sync();
+            for (int c0 = a0; c0 < a1; c0 += 64) {
+                const int p = c0 + lane;
+                int b0 = 0, len = 0;
+                if (p < a1) {
+                    const int K = Acol[p];
+                    b0 = Bptr[K];
+                    len = Bptr[K + 1] - b0;
+                }
+                const int incl = tsg_wave_incl_scan(len);
+                const int tot = __shfl(incl, 63, 64);
+                pre[lane] = incl - len;
+                pb0[lane] = b0;
+                tsg_wave_sync();
+                if (tot >= 64 * 32) {
+                    // long B tile-rows (32+ tiles on average): the 64 lanes sweep
+                    // one B tile-row at a time, no candidate search
+                    const int na = min(64, a1 - c0);
+                    for (int j = 0; j < na; j++) {
+                        const int bj = pb0[j], lj = (j + 1 < na ? pre[j + 1] : tot) - pre[j];
+                        for (int c = lane; c < lj; c += 64) {
+                            const int d = Bcol[bj + c] - w0;
+                            if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                        }
+                    }
+                } else {
+                    for (int c = lane; c < tot; c += 64) {
+                        int lo = 0;
+    #pragma unroll
+                        for (int st = 32; st > 0; st >>= 1)
+                            if (pre[lo + st] <= c) lo += st;
+                        const int d = Bcol[pb0[lo] + (c - pre[lo])] - w0;
                         if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
                     }
                 }
-            } else {
-                for (int c = lane; c < tot; c += 64) {
-                    int lo = 0;
-#pragma unroll
-                    for (int st = 32; st > 0; st >>= 1)
-                        if (pre[lo + st] <= c) lo += st;
-                    const int d = Bcol[pb0[lo] + (c - pre[lo])] - w0;
-                    if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
-                }
+                tsg_wave_sync();
             }
-            tsg_wave_sync();
+            if (!FILL && kept)
+                for (int k = lane; k < nw; k += 64) keep[k] = bm[k];
         }
         if (!FILL) {
             int cnt = 0;
@@ -1429,7 +1440,7 @@ extern "C" int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *A, const t
     if (A->tilem > 0) {
         hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
                            ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
-                           (int *)nullptr, (int *)nullptr);
+                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr);
         TSG_LAUNCH_CHECK();
     }
     long long tot = 0;
@@ -1459,17 +1470,38 @@ static TileOperands make_operands(const tsg_dtiles *A, const tsg_dtiles *B) {
     return op;
 }
 
-static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C, bool fill) {
+// Step-1 bitmap window in 32-bit words per wave: 256 (8192 tile columns,
+// 8 KiB of LDS per block) or 2048 (65536 tile columns, 32 KiB).  By default
+// the wide window serves B with 8193..65536 tile columns, where a row of a
+// product without band structure spans more than one narrow window; wider B
+// (banded operands such as stencils) keeps the narrow one.  The count pass's
+// bitmaps are kept for the fill pass up to 1 GiB.
+int tsg_s1_words(const tsg_dtiles *B, const tsg_tune &t) {
+    if (t.s1_window_words == 256 || t.s1_window_words == 2048) return t.s1_window_words;
+    return B->tilen > 8192 && B->tilen <= 65536 ? 2048 : 256;
+}
+static size_t keep_bytes(long long tilem, int words) {
+    const size_t b = (size_t)tilem * (size_t)words * 4;
+    return b <= ((size_t)1 << 30) ? b : 0;
+}
+
+static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C, bool fill,
+                        int words, unsigned *bmstore) {
     if (C->tilem <= 0) return TSG_OK;
-    if (!fill) {
-        hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0, s,
-                           A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, C->tile_ptr,
-                           (int *)nullptr, (int *)nullptr);
-    } else {
-        hipLaunchKernelGGL((k_step1<true, 256>), dim3(tsg_div_up(C->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0, s,
-                           A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, C->tile_ptr,
-                           C->tile_colidx, C->tile_rowidx);
-    }
+    const dim3 g(tsg_div_up(C->tilem, S1_WAVES)), blk(64 * S1_WAVES);
+    const bool wide = words == 2048;
+    if (!fill && !wide)
+        hipLaunchKernelGGL((k_step1<false, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
+                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore);
+    else if (!fill)
+        hipLaunchKernelGGL((k_step1<false, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
+                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore);
+    else if (!wide)
+        hipLaunchKernelGGL((k_step1<true, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
+                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore);
+    else
+        hipLaunchKernelGGL((k_step1<true, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
+                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
@@ -1483,6 +1515,7 @@ struct SpgemmWork {
     unsigned char *rowcls;
     int *cls_out;
     int4 *seg2;        // step-2 segments
+    unsigned *bmstore; // step-1 bitmaps kept from the count pass (may be NULL)
     int4 *seg3;        // step-3 element segments
     long long seg2_cap, seg3_cap;
     // heavy rows (synchronous pipeline only; NULL = none)
@@ -1498,23 +1531,28 @@ static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
 // Offsets of the work arrays inside one buffer; returns its size.  The
 // step-3 segments go last (the synchronous pipeline adds them only once
 // C's nonzero count is known).
-static size_t work_layout(long long tilem, long long numtile, long long nnz, size_t off[8]) {
-    const size_t sz[8] = {
+#define WORK_NARR 9
+
+static size_t work_layout(long long tilem, long long numtile, long long nnz, size_t off[WORK_NARR],
+                          int s1w = 256) {
+    const size_t sz[WORK_NARR] = {
         8 * (size_t)tsg_scan_partial_count(tilem), 8 * (size_t)tsg_scan_partial_count(numtile),
         8 * (size_t)tsg_scan_partial_count(tilem), 4 * ((size_t)tilem + 1), (size_t)tilem + 1, 32,
         16 * (size_t)seg_bound(tilem, numtile, G2_UMAX, 0, 0),
+        keep_bytes(tilem, s1w),  // step-1 bitmaps (enqueue-only pipeline)
         16 * (size_t)(tilem + nnz / (G3E_NNZ - 255) + 2)};
     size_t o = 0;
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < WORK_NARR; i++) {
         off[i] = o;
         o += up16(sz[i]);
     }
     return o;
 }
 
-static SpgemmWork work_at(void *base, long long tilem, long long numtile, long long nnz, void *seg3_base) {
-    size_t off[8];
-    work_layout(tilem, numtile, nnz, off);
+static SpgemmWork work_at(void *base, long long tilem, long long numtile, long long nnz, void *seg3_base,
+                          int s1w = 256) {
+    size_t off[WORK_NARR];
+    work_layout(tilem, numtile, nnz, off, s1w);
     char *b = (char *)base;
     SpgemmWork w;
     w.part1 = (long long *)(b + off[0]);
@@ -1524,7 +1562,8 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.rowcls = (unsigned char *)(b + off[4]);
     w.cls_out = (int *)(b + off[5]);
     w.seg2 = (int4 *)(b + off[6]);
-    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[7]);
+    w.bmstore = keep_bytes(tilem, s1w) ? (unsigned *)(b + off[7]) : nullptr;
+    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[8]);
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
     w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
     w.hflag = nullptr;
@@ -1673,7 +1712,15 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     C->tilen = B->tilen;
     // ---- step 1
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_ptr, (size_t)C->tilem + 1));
-    TSG_TRY(launch_step1(s, A, B, C, false));
+    if (tune.s1_window_words != 0 && tune.s1_window_words != 256 && tune.s1_window_words != 2048) {
+        tsg_set_error("spgemm: s1_window_words %d is not 0, 256 or 2048", tune.s1_window_words);
+        return TSG_ERR_ARG;
+    }
+    P.s1w = tsg_s1_words(B, tune);
+    unsigned *keep = nullptr;
+    const size_t kb = keep_bytes(C->tilem, P.s1w);
+    if (kb) TSG_TRY(tsg_alloc(ctx, (void **)&keep, kb));
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, keep));
     void *scr;
     TSG_TRY(tsg_alloc(ctx, &scr, 64));
     unsigned long long *bins = (unsigned long long *)scr;
@@ -1687,12 +1734,13 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     unsigned char *hflag = nullptr;
     int *hcnt = nullptr, *hlist = nullptr, *hspan = nullptr;
     double *hdense = nullptr;
-    size_t woff[8];
+    size_t woff[WORK_NARR];
     const size_t wbytes = work_layout(C->tilem, numtileC, 0, woff);
-    TSG_TRY(tsg_alloc(ctx, &wbuf, woff[7]));  // everything but the step-3 segments
+    TSG_TRY(tsg_alloc(ctx, &wbuf, woff[WORK_NARR - 1]));  // everything but the step-3 segments
     (void)wbytes;
     SpgemmWork w = work_at(wbuf, C->tilem, numtileC, 0, nullptr);
-    TSG_TRY(launch_step1(s, A, B, C, true));
+    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, keep));
+    tsg_free(ctx, keep);
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
     // ---- step 2
     const TileOperands op = make_operands(A, B);
@@ -1800,17 +1848,17 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     return tsg_spgemm_planned(ctx, A, B, tune_in, C, stats, nullptr);
 }
 
-size_t tsg_spgemm_work_bytes(const tsg_dtiles *C) {
-    size_t off[8];
-    return work_layout(C->tilem, C->numtile, C->nnz, off);
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, int s1w) {
+    size_t off[WORK_NARR];
+    return work_layout(C->tilem, C->numtile, C->nnz, off, s1w);
 }
 
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
                        const tsg_spgemm_plan &P, void *work) {
-    const SpgemmWork w = work_at(work, C->tilem, C->numtile, C->nnz, nullptr);
-    TSG_TRY(launch_step1(s, A, B, C, false));
+    const SpgemmWork w = work_at(work, C->tilem, C->numtile, C->nnz, nullptr, P.s1w);
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, w.bmstore));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, w.part1));
-    TSG_TRY(launch_step1(s, A, B, C, true));
+    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, w.bmstore));
     const TileOperands op = make_operands(A, B);
     TSG_TRY(launch_step2(s, op, B, C, P, w));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, w.part2));

@@ -101,13 +101,16 @@ struct tsg_spgemm_plan {
     int s3_elem;    // the tile-rows the row-wise kernel cannot hold go element-wise
     int elem_all;   // every tile-row goes element-wise
     int g3[6];      // row-wise step-3 LDS layout maxima
+    int s1w;        // step-1 bitmap window, 32-bit words per wave
 };
 int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune,
                        tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out);
 // Intermediate products of each A tile-row (device u64[A->tilem]).
 int tsg_tile_row_products(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, unsigned long long *d_out);
 // Device scratch bytes tsg_spgemm_enqueue needs for this C.
-size_t tsg_spgemm_work_bytes(const tsg_dtiles *C);
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, int s1w);
+// Step-1 bitmap window (32-bit words per wave) for products with this B.
+int tsg_s1_words(const tsg_dtiles *B, const tsg_tune &t);
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
                        const tsg_spgemm_plan &P, void *work);
 

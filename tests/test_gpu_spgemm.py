@@ -137,3 +137,24 @@ def test_repeat_is_deterministic(ctx):
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
     ctx.free(ta)
     ctx.free(da)
+
+
+def test_wide_step1_window_rmat(ctx):
+    """R-MAT scale 14 A*A^T, whose rows' candidates span the whole column
+    range: the check runs both step-1 windows (256 and 2048 words), all
+    arrays identical to the oracle."""
+    a = ts.gen_rmat(14, 8, seed=21)
+    spgemm_check(ctx, a, t(a), label="rmat14 AAT windows")
+
+
+def test_bad_step1_window_rejected(ctx):
+    a = ts.gen_stencil5(32, 32)
+    da = ctx.upload(a)
+    ta = ctx.csr2tile(da)
+    ctx.build_csc(ta)
+    tune = ts.default_tune()
+    tune.s1_window_words = 100
+    with pytest.raises(ts.TsgError):
+        ctx.spgemm(ta, ta, tune)
+    ctx.free(ta)
+    ctx.free(da)

@@ -76,6 +76,15 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
         assert np.array_equal(C2[k], C[k]), f"tile-parallel path differs in {k}"
     ctx.free(tc2)
+    # both step-1 bitmap windows give the same tile list
+    for words in (256, 2048):
+        tw = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
+        tw.s1_window_words = words
+        tcw, _ = ctx.spgemm(ta, tb, tw)
+        Cw = ctx.download_tiles(tcw)
+        for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "mask", "val"):
+            assert np.array_equal(Cw[k], C[k]), f"step-1 window {words} differs in {k}"
+        ctx.free(tcw)
     # element-wise numeric step with B's row index: for every tile-row, and
     # for the tile-rows the row-wise tile kernel cannot hold
     ctx.build_rowindex(tb)
