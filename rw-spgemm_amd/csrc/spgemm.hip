@@ -516,26 +516,43 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                 if (ow + st < G2_THREADS && s_tex[ow + st] <= x) ow += st;
             const int tt = tc + ow;
             int j = x - s_tex[ow];
-            const unsigned short *am16 = op.Amask + (long long)tt * 16;
+            // the tile's 16 row masks in one load; the row holding its j-th entry
+            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)tt * 16);
+            const uint4 mA = am4[0], mB = am4[1];
+            const unsigned mw[8] = {mA.x, mA.y, mA.z, mA.w, mB.x, mB.y, mB.z, mB.w};
             int r = 0;
-            unsigned am = am16[0];
-            while (r < 15 && j >= __popc(am)) {  // the row holding the j-th entry of the tile
-                j -= __popc(am);
-                am = am16[++r];
+            unsigned am = mw[0] & 0xffffu;
+#pragma unroll
+            for (int rr = 0; rr < 16; rr++) {
+                const unsigned mr = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
+                if (rr == r && r < 15 && j >= __popc(mr)) {
+                    j -= __popc(mr);
+                    r++;
+                }
             }
+            am = (mw[r >> 1] >> ((r & 1) * 16)) & 0xffffu;
             for (int q = 0; q < j; q++) am &= ~(1u << (31 - __clz(am)));
             const int k = op.Acol[tt] * 16 + 15 - (31 - __clz(am));
             const int e1 = Browp[k + 1];
-            for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); e < e1; e++) {
-                const int col = Bcolx[e];
-                if (col >= col_hi) break;
-                const int J = col >> 4;
-                unsigned h = col_hash10(J);  // J is in the segment's tile list
-                int probe = 0;
-                while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
-                if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
-                const unsigned bit = 1u << (15 - (col & 15));
-                atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
+            bool more = true;
+            // B row k's entries in the segment's column range, 4 loads at a time
+            for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); more && e < e1; e += 4) {
+                int cols[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) cols[v] = e + v < e1 ? Bcolx[e + v] : 0x7fffffff;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int col = cols[v];
+                    if (col >= col_hi) more = false;
+                    if (!more) continue;
+                    const int J = col >> 4;
+                    unsigned h = col_hash10(J);  // J is in the segment's tile list
+                    int probe = 0;
+                    while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
+                    if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
+                    const unsigned bit = 1u << (15 - (col & 15));
+                    atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
+                }
             }
         }
         __syncthreads();  // s_tex is rewritten by the next chunk
@@ -856,38 +873,62 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int gb = tid & 48;  // first lane of this group in its wave
     __shared__ int s_gex[16][16];             // per group: entries before each lane's tile
     __shared__ unsigned short s_gam[16][16];  // per group: each lane's tile row mask
+    // adds a * b into C entry (row 16I+r, col), located through the hash
+    // (the entry is in C's structure; the probe bound is not reached for a
+    // consistent C)
+    auto acc_add = [&](int col, double a, double b) {
+        const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
+        unsigned h = ent_hash(key);
+        int probe = 0;
+        while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
+        if (probe < G3E_EHASH) {
+            double *acc = &s_acc[s_epos[h]];
+            *acc = __builtin_fma(a, b, *acc);
+        }
+    };
     if (row < m && nseg_nnz > 0 && op.Arowp && !skip) {
-        // A's row index: row 16I+r's entries are contiguous, ascending k; the
-        // 16 lanes take 16 of them and find their B ranges in parallel
+        // A's row index: row 16I+r's entries are contiguous, ascending k.  The
+        // 16 lanes take 16 of them (sources) and find their B ranges in
+        // parallel; then the first 16 in-range B entries of all 16 sources are
+        // loaded at once (lane l holds entry l of each), so the ordered
+        // accumulation below waits on no global load for rows of up to 16
+        // entries in the segment's column range.
         const int x1 = op.Arowp[row + 1];
         for (int x0 = op.Arowp[row]; x0 < x1; x0 += 16) {
             const int x = x0 + l;
             int e0 = 0, e1 = 0;
             double a = 0.0;
-            bool have = false;
             if (x < x1) {
                 const int k = op.Acolx[x];
                 a = op.Avalx[x];
                 e1 = Browp[k + 1];
                 e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
-                have = e0 < e1 && Bcolx[e0] < col_hi;
             }
-            unsigned hv = (unsigned)(__ballot(have) >> gb) & 0xffffu;
-            while (hv) {  // entries with B entries in range, ascending k
-                const int src = __ffs(hv) - 1;
-                hv &= hv - 1;
-                const double as = __shfl(a, gb + src, 64);
-                const int es0 = __shfl(e0, gb + src, 64), es1 = __shfl(e1, gb + src, 64);
-                for (int e = es0 + l; e < es1; e += 16) {
-                    const int col = Bcolx[e];
-                    if (col >= col_hi) break;
-                    const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
-                    unsigned h = ent_hash(key);  // the entry is in C's structure
-                    int probe = 0;
-                    while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
-                    if (probe == G3E_EHASH) continue;  // not reached for a consistent C structure
-                    double *acc = &s_acc[s_epos[h]];
-                    *acc = __builtin_fma(as, Bvalx[e], *acc);
+            int pc[16];
+            double pv[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int qs0 = __shfl(e0, gb + q, 64), qs1 = __shfl(e1, gb + q, 64);
+                const int e = qs0 + l;
+                pc[q] = 0x7fffffff;
+                pv[q] = 0.0;
+                if (e < qs1) {
+                    pc[q] = Bcolx[e];
+                    pv[q] = Bvalx[e];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 16; q++) {  // sources in ascending k
+                const double as = __shfl(a, gb + q, 64);
+                const int qs0 = __shfl(e0, gb + q, 64), qs1 = __shfl(e1, gb + q, 64);
+                if (pc[q] < col_hi) acc_add(pc[q], as, pv[q]);
+                // entries past the first 16 (the last prefetched one in range)
+                if (qs1 - qs0 > 16 && __shfl(pc[q], gb + 15, 64) < col_hi) {
+                    for (int e = qs0 + 16 + l; e < qs1; e += 16) {
+                        const int col = Bcolx[e];
+                        if (col >= col_hi) break;
+                        acc_add(col, as, Bvalx[e]);
+                    }
                 }
                 tsg_wave_sync();  // the next k of this row reads what lanes wrote
             }
@@ -942,13 +983,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     for (int e = es0 + l; e < es1; e += 16) {
                         const int col = Bcolx[e];
                         if (col >= col_hi) break;
-                        const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
-                        unsigned h = ent_hash(key);  // the entry is in C's structure
-                        int probe = 0;
-                        while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
-                        if (probe == G3E_EHASH) continue;  // not reached for a consistent C structure
-                        double *acc = &s_acc[s_epos[h]];
-                        *acc = __builtin_fma(as, Bvalx[e], *acc);
+                        acc_add(col, as, Bvalx[e]);
                     }
                     tsg_wave_sync();  // the next k of this row reads what lanes wrote
                 }
@@ -1052,14 +1087,51 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
         const int I = i >> 4, r = i & 15;
         const int c0 = Cptr[I], c1 = Cptr[I + 1];
         const int colbase = Ccol[c0] * 16;
-        for (int x = Arowp[i]; x < Arowp[i + 1]; x++) {  // ascending k
-            const int k = Acolx[x];
-            const double a = Avalx[x];
-            for (int e = Browp[k] + lane; e < Browp[k + 1]; e += 64) {  // distinct columns
-                double *d = row_acc + (Bcolx[e] - colbase);
-                wg_store(d, __builtin_fma(a, Bvalx[e], wg_load(d)));
+        const int xe = Arowp[i + 1];
+        for (int xc = Arowp[i]; xc < xe; xc += 64) {
+            // 64 A entries (sources) at a time: lane q holds source q's value
+            // and B row range
+            const int n = min(64, xe - xc);
+            int qb0 = 0, qb1 = 0;
+            double qa = 0.0;
+            if (lane < n) {
+                const int k = Acolx[xc + lane];
+                qa = Avalx[xc + lane];
+                qb0 = Browp[k];
+                qb1 = Browp[k + 1];
             }
-            wg_fence();  // the next k reads what this one wrote
+            // the first 64 entries of the next source are loaded before this
+            // source's accumulation, so each k waits on the accumulator only
+            int e0 = __shfl(qb0, 0, 64), e1 = __shfl(qb1, 0, 64);
+            int col = -1;
+            double bv = 0.0;
+            if (e0 + lane < e1) {
+                col = Bcolx[e0 + lane];
+                bv = Bvalx[e0 + lane];
+            }
+            for (int q = 0; q < n; q++) {  // ascending k
+                const double a = __shfl(qa, q, 64);
+                const int f0 = __shfl(qb0, (q + 1) & 63, 64), f1 = __shfl(qb1, (q + 1) & 63, 64);
+                int ncol = -1;
+                double nbv = 0.0;
+                if (q + 1 < n && f0 + lane < f1) {
+                    ncol = Bcolx[f0 + lane];
+                    nbv = Bvalx[f0 + lane];
+                }
+                if (col >= 0) {  // distinct columns within one B row
+                    double *d = row_acc + (col - colbase);
+                    wg_store(d, __builtin_fma(a, bv, wg_load(d)));
+                }
+                for (int e = e0 + 64 + lane; e < e1; e += 64) {
+                    double *d = row_acc + (Bcolx[e] - colbase);
+                    wg_store(d, __builtin_fma(a, Bvalx[e], wg_load(d)));
+                }
+                wg_fence();  // the next k reads what this one wrote
+                col = ncol;
+                bv = nbv;
+                e0 = f0;
+                e1 = f1;
+            }
         }
         for (int t = c0 + lane; t < c1; t += 64) {  // into C, clearing the dense row
             unsigned cm = Cmask[(long long)t * 16 + r];
