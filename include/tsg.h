@@ -172,7 +172,9 @@ typedef struct {
                               (default 0) */
     int heavy_min_products; /* a row with more intermediate products (or more
                               than 2048 A entries) is heavy (0 = 65536) */
-    int reserved[1];
+    int heavy_item_products; /* a heavy row is split into column parts of
+                              4096 columns, handed out in runs of about this
+                              many products (0 = 262144) */
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 

@@ -1012,139 +1012,165 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
 }
 
 // ------------------------------------------------------------------ step 3, heavy rows
-// Rows with very many intermediate products (hub rows of power-law
-// graphs) would make every C segment of their tile-row re-walk the whole
-// A row.  They get one wave each instead: the wave walks A row i once in
-// ascending k, its lanes accumulating B row k into a dense FP64 row in
-// global memory (one per wave, spanning the C tile-row's columns), then
-// scatters the row into C's final positions and clears what it touched.
-// Only lanes of one wave share a dense row, so workgroup-scope atomics and
-// fences order the k steps (no cache maintenance is needed within a CU).
+// Rows with very many intermediate products (hub rows of power-law graphs)
+// would make every C segment of their tile-row re-walk the whole A row.
+// They are computed one wave at a time instead, in column parts of
+// HEAVY_PART_COLS columns whose FP64 accumulators live in the wave's own
+// LDS window: for part [lo, lo + HEAVY_PART_COLS) the wave walks A row i in
+// ascending k, its lanes adding a_ik * b_kj for the entries of B row k in
+// the part (distinct columns per k; LDS instructions of one wave complete
+// in order, so the next k reads what this one wrote), then copies the part
+// into C's final positions.  A work item is a run of consecutive parts of
+// one row; between parts the wave keeps, per A entry, the position in
+// B row k where the next part starts (a cursor in global scratch), so only
+// an item's first part searches B's rows.
 #define HEAVY_MIN_PRODUCTS 65536
-#define HEAVY_MAX_WAVES 1024
+#define HEAVY_PART_COLS 4096         // 32 KiB of LDS per wave
+#define HEAVY_ITEM_PRODUCTS 262144   // products per work item, about
+#define HEAVY_MAX_WAVES 4096
+#define HEAVY_CURSOR_BYTES (512ll << 20)
 
+// Per C row of an element-wise tile-row: heavy flag and work items (runs of
+// parts); info[0] = longest heavy A row, info[1] = heavy rows.
 __global__ void __launch_bounds__(256)
 k_heavy_flags(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const int *__restrict__ Browp,
               const unsigned char *__restrict__ rowcls, const int *__restrict__ Cptr, const int *__restrict__ Ccol,
-              int m, long long min_products, unsigned char *__restrict__ flag, int *__restrict__ cnt,
-              int *__restrict__ span) {
+              int m, long long min_products, long long item_products, unsigned char *__restrict__ flag,
+              int *__restrict__ cnt, int *__restrict__ info) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    int h = 0;
+    int h = 0, items = 0;
     if (i < m && rowcls[i >> 4] == 0) {
         const int x0 = Arowp[i], x1 = Arowp[i + 1];
+        long long p = -1;  // unknown for long A rows
         if (x1 - x0 > 2048) {
             h = 1;
         } else {
-            long long p = 0;
+            p = 0;
             for (int x = x0; x < x1; x++) p += Browp[Acolx[x] + 1] - Browp[Acolx[x]];
             h = p > min_products;
         }
         if (h) {
             const int I = i >> 4, c0 = Cptr[I], c1 = Cptr[I + 1];
-            atomicMax(span, (Ccol[c1 - 1] - Ccol[c0] + 1) * 16);
+            const long long span = (long long)(Ccol[c1 - 1] - Ccol[c0] + 1) * 16;
+            const int nparts = (int)((span + HEAVY_PART_COLS - 1) / HEAVY_PART_COLS);
+            items = p < 0 ? nparts
+                          : (int)min<long long>(nparts, max<long long>(1, (p + item_products - 1) / item_products));
+            atomicMax(&info[0], x1 - x0);
+            atomicAdd(&info[1], 1);
         }
     }
     if (i < m) {
         flag[i] = (unsigned char)h;
-        cnt[i] = h;
+        cnt[i] = items;
     }
 }
 
+// Work items {row, first part | end part << 16}, parts split evenly.
 __global__ void __launch_bounds__(256)
-k_heavy_list(const int *__restrict__ off, const unsigned char *__restrict__ flag, int m, int *__restrict__ list) {
+k_heavy_list(const int *__restrict__ off, const unsigned char *__restrict__ flag, const int *__restrict__ Cptr,
+             const int *__restrict__ Ccol, int m, int2 *__restrict__ items) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < m && flag[i]) list[off[i]] = i;
+    if (i >= m || !flag[i]) return;
+    const int I = i >> 4, c0 = Cptr[I], c1 = Cptr[I + 1];
+    const long long span = (long long)(Ccol[c1 - 1] - Ccol[c0] + 1) * 16;
+    const int nparts = (int)((span + HEAVY_PART_COLS - 1) / HEAVY_PART_COLS);
+    const int o = off[i], n = off[i + 1] - o;
+    for (int g = 0; g < n; g++) {
+        const int p0 = (int)((long long)nparts * g / n), p1 = (int)((long long)nparts * (g + 1) / n);
+        items[o + g] = make_int2(i, (int)((unsigned)p0 | ((unsigned)p1 << 16)));
+    }
 }
 
-__device__ __forceinline__ double wg_load(double *p) {
-    const unsigned long long v =
-        __hip_atomic_load(reinterpret_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    return __longlong_as_double((long long)v);
-}
-__device__ __forceinline__ void wg_store(double *p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void wg_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const double *__restrict__ Avalx,
               const int *__restrict__ Browp, const int *__restrict__ Bcolx, const double *__restrict__ Bvalx,
-              const int *__restrict__ hlist, const int *__restrict__ nheavy, const int *__restrict__ Cptr,
+              const int2 *__restrict__ items, const int *__restrict__ nitems, const int *__restrict__ Cptr,
               const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
               const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, double *__restrict__ Cval,
-              double *__restrict__ dense, long long span) {
-    const int lane = threadIdx.x & 63;
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
-    double *row_acc = dense + (long long)wv * span;
-    const int nh = *nheavy;
-    for (int h = wv; h < nh; h += nw) {
-        const int i = hlist[h];
+              int *__restrict__ cursor, long long cur_stride) {
+    __shared__ double s_acc[HEAVY_PART_COLS];
+    const int lane = threadIdx.x;
+    int *cur = cursor + (long long)blockIdx.x * cur_stride;
+    const int ni = *nitems;
+    for (int h = blockIdx.x; h < ni; h += gridDim.x) {
+        const int2 it = items[h];
+        const int i = it.x, p0 = (int)((unsigned)it.y & 0xffffu), p1 = (int)((unsigned)it.y >> 16);
         const int I = i >> 4, r = i & 15;
         const int c0 = Cptr[I], c1 = Cptr[I + 1];
         const int colbase = Ccol[c0] * 16;
-        const int xe = Arowp[i + 1];
-        for (int xc = Arowp[i]; xc < xe; xc += 64) {
-            // 64 A entries (sources) at a time: lane q holds source q's value
-            // and B row range
-            const int n = min(64, xe - xc);
-            int qb0 = 0, qb1 = 0;
-            double qa = 0.0;
-            if (lane < n) {
-                const int k = Acolx[xc + lane];
-                qa = Avalx[xc + lane];
-                qb0 = Browp[k];
-                qb1 = Browp[k + 1];
-            }
-            // the first 64 entries of the next source are loaded before this
-            // source's accumulation, so each k waits on the accumulator only
-            int e0 = __shfl(qb0, 0, 64), e1 = __shfl(qb1, 0, 64);
-            int col = -1;
-            double bv = 0.0;
-            if (e0 + lane < e1) {
-                col = Bcolx[e0 + lane];
-                bv = Bvalx[e0 + lane];
-            }
-            for (int q = 0; q < n; q++) {  // ascending k
-                const double a = __shfl(qa, q, 64);
-                const int f0 = __shfl(qb0, (q + 1) & 63, 64), f1 = __shfl(qb1, (q + 1) & 63, 64);
-                int ncol = -1;
-                double nbv = 0.0;
-                if (q + 1 < n && f0 + lane < f1) {
-                    ncol = Bcolx[f0 + lane];
-                    nbv = Bvalx[f0 + lane];
+        const int xa = Arowp[i], xe = Arowp[i + 1];
+        for (int part = p0; part < p1; part++) {
+            const int lo = colbase + part * HEAVY_PART_COLS, hi = lo + HEAVY_PART_COLS;
+            for (int j = lane; j < HEAVY_PART_COLS; j += 64) s_acc[j] = 0.0;
+            tsg_wave_sync();
+            for (int xc = xa; xc < xe; xc += 64) {
+                // 64 A entries (sources): lane q holds source q's value and the
+                // range of B row k from the part's first column on
+                const int n = min(64, xe - xc);
+                int qb0 = 0, qb1 = 0;
+                double qa = 0.0;
+                if (lane < n) {
+                    const int k = Acolx[xc + lane];
+                    qa = Avalx[xc + lane];
+                    qb1 = Browp[k + 1];
+                    qb0 = part == p0 ? lower_bound_i(Bcolx, Browp[k], qb1, lo) : cur[xc - xa + lane];
                 }
-                if (col >= 0) {  // distinct columns within one B row
-                    double *d = row_acc + (col - colbase);
-                    wg_store(d, __builtin_fma(a, bv, wg_load(d)));
+                // the first 64 entries of the next source are loaded before
+                // this source's accumulation
+                int e0 = __shfl(qb0, 0, 64), e1 = __shfl(qb1, 0, 64);
+                int col = 0x7fffffff;
+                double bv = 0.0;
+                if (e0 + lane < e1) {
+                    col = Bcolx[e0 + lane];
+                    bv = Bvalx[e0 + lane];
                 }
-                for (int e = e0 + 64 + lane; e < e1; e += 64) {
-                    double *d = row_acc + (Bcolx[e] - colbase);
-                    wg_store(d, __builtin_fma(a, Bvalx[e], wg_load(d)));
+                int next = qb0;  // lane q: where source q's next part starts
+                for (int q = 0; q < n; q++) {  // ascending k
+                    const double a = __shfl(qa, q, 64);
+                    const int f0 = __shfl(qb0, (q + 1) & 63, 64), f1 = __shfl(qb1, (q + 1) & 63, 64);
+                    int ncol = 0x7fffffff;
+                    double nbv = 0.0;
+                    if (q + 1 < n && f0 + lane < f1) {
+                        ncol = Bcolx[f0 + lane];
+                        nbv = Bvalx[f0 + lane];
+                    }
+                    const bool in = col < hi;  // in-range entries form a prefix
+                    if (in) s_acc[col - lo] = __builtin_fma(a, bv, s_acc[col - lo]);
+                    int e = e0 + __popcll(__ballot(in));
+                    if (e == e0 + 64) {  // more than 64 entries in the part
+                        while (e < e1) {
+                            const int ee = e + lane;
+                            const int c = ee < e1 ? Bcolx[ee] : 0x7fffffff;
+                            const bool in2 = c < hi;
+                            if (in2) s_acc[c - lo] = __builtin_fma(a, Bvalx[ee], s_acc[c - lo]);
+                            const int cnt = __popcll(__ballot(in2));
+                            e += cnt;
+                            if (cnt < 64) break;
+                        }
+                    }
+                    if (lane == q) next = e;
+                    tsg_wave_sync();  // the next k reads what this one wrote
+                    col = ncol;
+                    bv = nbv;
+                    e0 = f0;
+                    e1 = f1;
                 }
-                wg_fence();  // the next k reads what this one wrote
-                col = ncol;
-                bv = nbv;
-                e0 = f0;
-                e1 = f1;
+                if (part + 1 < p1 && lane < n) cur[xc - xa + lane] = next;
             }
+            // the part's C entries of row i, into their final positions
+            const int t0 = lower_bound_i(Ccol, c0, c1, lo >> 4), t1 = lower_bound_i(Ccol, t0, c1, hi >> 4);
+            for (int t = t0 + lane; t < t1; t += 64) {
+                unsigned cm = Cmask[(long long)t * 16 + r];
+                long long pos = (long long)Cnnz[t] + Cptr16[(long long)t * 16 + r];
+                const double *d = s_acc + (Ccol[t] * 16 - lo);
+                while (cm) {
+                    const int bb = 31 - __clz(cm);
+                    cm &= ~(1u << bb);
+                    Cval[pos++] = d[15 - bb];
+                }
+            }
+            tsg_wave_sync();  // the next part clears the window
         }
-        for (int t = c0 + lane; t < c1; t += 64) {  // into C, clearing the dense row
-            unsigned cm = Cmask[(long long)t * 16 + r];
-            long long pos = (long long)Cnnz[t] + Cptr16[(long long)t * 16 + r];
-            double *d = row_acc + (Ccol[t] * 16 - colbase);
-            while (cm) {
-                const int bb = 31 - __clz(cm);
-                cm &= ~(1u << bb);
-                Cval[pos++] = wg_load(d + 15 - bb);
-                wg_store(d + 15 - bb, 0.0);
-            }
-        }
-        wg_fence();
     }
 }
 
@@ -1592,10 +1618,11 @@ struct SpgemmWork {
     long long seg2_cap, seg3_cap;
     // heavy rows (synchronous pipeline only; NULL = none)
     unsigned char *hflag;  // per C row
-    int *hlist, *hcount;   // heavy rows, their number (device)
-    int nheavy, hwaves;
-    double *dense;         // hwaves dense rows of hspan doubles
-    long long hspan;
+    int2 *hitems;          // heavy-row work items
+    int *hcount;           // their number (device)
+    int nheavy, nitems, hwaves;
+    int *cursor;           // hwaves cursor arrays of hstride ints
+    long long hstride;
 };
 
 static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -1639,10 +1666,11 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
     w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
     w.hflag = nullptr;
-    w.hlist = w.hcount = nullptr;
-    w.nheavy = w.hwaves = 0;
-    w.dense = nullptr;
-    w.hspan = 0;
+    w.hitems = nullptr;
+    w.hcount = nullptr;
+    w.nheavy = w.nitems = w.hwaves = 0;
+    w.cursor = nullptr;
+    w.hstride = 0;
     return w;
 }
 
@@ -1722,11 +1750,11 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
                            C->tile_nnz, C->csr_col, C->val, (const unsigned char *)w.hflag);
         TSG_LAUNCH_CHECK();
-        if (w.hflag && w.nheavy > 0) {
-            hipLaunchKernelGGL(k_step3_heavy, dim3(tsg_div_up(w.hwaves, 4)), dim3(256), 0, s, op.Arowp, op.Acolx,
-                               op.Avalx, (const int *)B->rowx_ptr, (const int *)B->rowx_col,
-                               (const double *)B->rowx_val, (const int *)w.hlist, (const int *)w.hcount, C->tile_ptr,
-                               C->tile_colidx, C->mask, C->csr_ptr, C->tile_nnz, C->val, w.dense, w.hspan);
+        if (w.hflag && w.nitems > 0) {
+            hipLaunchKernelGGL(k_step3_heavy, dim3(w.hwaves), dim3(64), 0, s, op.Arowp, op.Acolx, op.Avalx,
+                               (const int *)B->rowx_ptr, (const int *)B->rowx_col, (const double *)B->rowx_val,
+                               (const int2 *)w.hitems, (const int *)w.hcount, C->tile_ptr, C->tile_colidx, C->mask,
+                               C->csr_ptr, C->tile_nnz, C->val, w.cursor, w.hstride);
             TSG_LAUNCH_CHECK();
         }
     } else if (P.all || P.s3_unfit) {
@@ -1804,8 +1832,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
     void *wbuf = nullptr, *w3buf = nullptr;
     unsigned char *hflag = nullptr;
-    int *hcnt = nullptr, *hlist = nullptr, *hspan = nullptr;
-    double *hdense = nullptr;
+    int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr;
+    int2 *hitems = nullptr;
     size_t woff[WORK_NARR];
     const size_t wbytes = work_layout(C->tilem, numtileC, 0, woff);
     TSG_TRY(tsg_alloc(ctx, &wbuf, woff[WORK_NARR - 1]));  // everything but the step-3 segments
@@ -1851,33 +1879,42 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                 const int m = C->m;
                 TSG_TRY(tsg_alloc_t(ctx, &hflag, (size_t)m + 16));
                 TSG_TRY(tsg_alloc_t(ctx, &hcnt, (size_t)m + 1));
-                TSG_TRY(tsg_alloc_t(ctx, &hspan, 4));
-                TSG_HIP(hipMemsetAsync(hspan, 0, 4, s));
+                TSG_TRY(tsg_alloc_t(ctx, &hinfo, 2));
+                TSG_HIP(hipMemsetAsync(hinfo, 0, 8, s));
                 hipLaunchKernelGGL(k_heavy_flags, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, op.Arowp, op.Acolx,
                                    (const int *)B->rowx_ptr, (const unsigned char *)w.rowcls, C->tile_ptr,
                                    C->tile_colidx, m,
                                    (long long)(tune.heavy_min_products > 0 ? tune.heavy_min_products
                                                                            : HEAVY_MIN_PRODUCTS),
-                                   hflag, hcnt, hspan);
+                                   (long long)(tune.heavy_item_products > 0 ? tune.heavy_item_products
+                                                                            : HEAVY_ITEM_PRODUCTS),
+                                   hflag, hcnt, hinfo);
                 TSG_LAUNCH_CHECK();
-                long long nh = 0;
-                TSG_TRY(tsg_scan_i32(ctx, hcnt, hcnt, m, &nh));
-                if (nh > 0) {
-                    long long span = 0;
-                    TSG_TRY(tsg_read_scalar_i64(ctx, hspan, 0, &span));
-                    TSG_TRY(tsg_alloc_t(ctx, &hlist, (size_t)nh));
+                long long ni = 0;
+                TSG_TRY(tsg_scan_i32(ctx, hcnt, hcnt, m, &ni));
+                if (ni > 0x7fffffffLL) {
+                    tsg_set_error("spgemm: %lld heavy-row work items exceed the int32 range", ni);
+                    return TSG_ERR_ARG;
+                }
+                if (ni > 0) {
+                    long long info = 0;  // {longest heavy A row, heavy rows}
+                    TSG_TRY(tsg_read_scalar_i64(ctx, hinfo, 1, &info));
+                    const long long maxdeg = info & 0xffffffffLL, nrows = info >> 32;
+                    TSG_TRY(tsg_alloc_t(ctx, &hitems, (size_t)ni));
                     hipLaunchKernelGGL(k_heavy_list, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, (const int *)hcnt,
-                                       (const unsigned char *)hflag, m, hlist);
+                                       (const unsigned char *)hflag, C->tile_ptr, C->tile_colidx, m, hitems);
                     TSG_LAUNCH_CHECK();
-                    w.hwaves = (int)std::min<long long>(nh, HEAVY_MAX_WAVES);
-                    TSG_TRY(tsg_alloc_t(ctx, &hdense, (size_t)w.hwaves * (size_t)span));
-                    TSG_HIP(hipMemsetAsync(hdense, 0, (size_t)w.hwaves * (size_t)span * 8, s));
+                    // one wave per item up to HEAVY_MAX_WAVES, cursors within HEAVY_CURSOR_BYTES
+                    const long long by_mem = std::max<long long>(256, HEAVY_CURSOR_BYTES / (4 * std::max(1LL, maxdeg)));
+                    w.hwaves = (int)std::min<long long>(ni, std::min<long long>(HEAVY_MAX_WAVES, by_mem));
+                    w.hstride = std::max(1LL, maxdeg);
+                    TSG_TRY(tsg_alloc_t(ctx, &hcur, (size_t)w.hwaves * (size_t)w.hstride));
                     w.hflag = hflag;
-                    w.hlist = hlist;
+                    w.hitems = hitems;
                     w.hcount = hcnt + m;
-                    w.nheavy = (int)nh;
-                    w.dense = hdense;
-                    w.hspan = span;
+                    w.nheavy = (int)nrows;
+                    w.nitems = (int)ni;
+                    w.cursor = hcur;
                 }
             }
         }
@@ -1900,12 +1937,12 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         stats->numtileC = numtileC;
         stats->nnzC = nnzC;
         stats->heavy_rows = w.nheavy;
-        stats->heavy_bytes = (long long)w.hwaves * w.hspan * 8 + (w.nheavy ? 9LL * C->m : 0);
+        stats->heavy_bytes = (long long)w.hwaves * w.hstride * 4 + (w.nheavy ? 5LL * C->m + 8LL * w.nitems : 0);
         for (auto &e : ev) hipEventDestroy(e);
     }
-    tsg_free(ctx, hdense);
-    tsg_free(ctx, hlist);
-    tsg_free(ctx, hspan);
+    tsg_free(ctx, hcur);
+    tsg_free(ctx, hitems);
+    tsg_free(ctx, hinfo);
     tsg_free(ctx, hcnt);
     tsg_free(ctx, hflag);
     tsg_free(ctx, w3buf);

@@ -79,7 +79,7 @@ class HTiles(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("dense_acc_min_nnz", _i), ("s1_window_words", _i), ("force_tile_parallel", _i),
                 ("batch_group_nnz_cap", _i), ("element_mode", _i), ("no_heavy_rows", _i),
-                ("heavy_min_products", _i), ("reserved", _i * 1)]
+                ("heavy_min_products", _i), ("heavy_item_products", _i)]
 
 
 class Stats(C.Structure):

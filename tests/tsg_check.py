@@ -89,19 +89,22 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     # for the tile-rows the row-wise tile kernel cannot hold
     ctx.build_rowindex(tb)
     # with A's row index too, and rows above 64 products computed one wave
-    # each (the heavy-row path)
-    configs = [(2, 0), (0, 0)]
+    # each (the heavy-row path: column parts of 4096 columns in LDS, one run
+    # each; with items of ~256 products, runs of column parts go to
+    # different waves and each run's later parts start from the cursors)
+    configs = [(2, 0, 0), (0, 0, 0)]
     if b is not None:
         ctx.build_rowindex(ta)
-    configs.append((2, 64))
-    for mode, heavy in configs:
+    configs += [(2, 64, 0), (2, 64, 256)]
+    for mode, heavy, item in configs:
         t3 = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
         t3.element_mode = mode
         t3.heavy_min_products = heavy
+        t3.heavy_item_products = item
         tc3, _ = ctx.spgemm(ta, tb, t3)
         C3 = ctx.download_tiles(tc3)
         for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
-            assert np.array_equal(C3[k], C[k]), f"element mode {mode} / heavy {heavy} differs in {k}"
+            assert np.array_equal(C3[k], C[k]), f"element mode {mode} / heavy {heavy} / item {item} differs in {k}"
         ctx.free(tc3)
     for obj in (dc, tc):
         ctx.free(obj)
