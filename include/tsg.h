@@ -174,7 +174,7 @@ typedef struct {
                               than 2048 A entries) is heavy (0 = 65536) */
     int heavy_item_products; /* a heavy row is split into column parts of
                               4096 columns, handed out in runs of about this
-                              many products (0 = 262144) */
+                              many products (0 = 32768) */
 } tsg_tune;
 void tsg_tune_default(tsg_tune *t);
 

@@ -1025,8 +1025,10 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
 // B row k where the next part starts (a cursor in global scratch), so only
 // an item's first part searches B's rows.
 #define HEAVY_MIN_PRODUCTS 65536
+#ifndef HEAVY_PART_COLS
 #define HEAVY_PART_COLS 4096         // 32 KiB of LDS per wave
-#define HEAVY_ITEM_PRODUCTS 262144   // products per work item, about
+#endif
+#define HEAVY_ITEM_PRODUCTS 32768    // products per work item, about
 #define HEAVY_MAX_WAVES 4096
 #define HEAVY_CURSOR_BYTES (512ll << 20)
 
@@ -1081,6 +1083,10 @@ k_heavy_list(const int *__restrict__ off, const unsigned char *__restrict__ flag
     }
 }
 
+#ifndef HEAVY_STAGE
+#define HEAVY_STAGE 8  // B entries per source staged in LDS per part
+#endif
+
 __global__ void __launch_bounds__(64)
 k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const double *__restrict__ Avalx,
               const int *__restrict__ Browp, const int *__restrict__ Bcolx, const double *__restrict__ Bvalx,
@@ -1089,6 +1095,10 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
               const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, double *__restrict__ Cval,
               int *__restrict__ cursor, long long cur_stride) {
     __shared__ double s_acc[HEAVY_PART_COLS];
+    __shared__ double s_sv[64 * HEAVY_STAGE];  // staged B values, source-major
+    __shared__ int s_sc[64 * HEAVY_STAGE];     // their columns
+    __shared__ double s_a[64];
+    __shared__ int s_n[64];  // staged entries in the part | more may follow << 8
     const int lane = threadIdx.x;
     int *cur = cursor + (long long)blockIdx.x * cur_stride;
     const int ni = *nitems;
@@ -1102,58 +1112,87 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
         for (int part = p0; part < p1; part++) {
             const int lo = colbase + part * HEAVY_PART_COLS, hi = lo + HEAVY_PART_COLS;
             for (int j = lane; j < HEAVY_PART_COLS; j += 64) s_acc[j] = 0.0;
-            tsg_wave_sync();
             for (int xc = xa; xc < xe; xc += 64) {
-                // 64 A entries (sources): lane q holds source q's value and the
-                // range of B row k from the part's first column on
+                // 64 A entries (sources) at once: lane q finds where B row k
+                // enters the part and stages its first HEAVY_STAGE entries
                 const int n = min(64, xe - xc);
-                int qb0 = 0, qb1 = 0;
-                double qa = 0.0;
+                int cb = 0, ce = 0, cnt = 0;
+                bool more = false;
                 if (lane < n) {
                     const int k = Acolx[xc + lane];
-                    qa = Avalx[xc + lane];
-                    qb1 = Browp[k + 1];
-                    qb0 = part == p0 ? lower_bound_i(Bcolx, Browp[k], qb1, lo) : cur[xc - xa + lane];
-                }
-                // the first 64 entries of the next source are loaded before
-                // this source's accumulation
-                int e0 = __shfl(qb0, 0, 64), e1 = __shfl(qb1, 0, 64);
-                int col = 0x7fffffff;
-                double bv = 0.0;
-                if (e0 + lane < e1) {
-                    col = Bcolx[e0 + lane];
-                    bv = Bvalx[e0 + lane];
-                }
-                int next = qb0;  // lane q: where source q's next part starts
-                for (int q = 0; q < n; q++) {  // ascending k
-                    const double a = __shfl(qa, q, 64);
-                    const int f0 = __shfl(qb0, (q + 1) & 63, 64), f1 = __shfl(qb1, (q + 1) & 63, 64);
-                    int ncol = 0x7fffffff;
-                    double nbv = 0.0;
-                    if (q + 1 < n && f0 + lane < f1) {
-                        ncol = Bcolx[f0 + lane];
-                        nbv = Bvalx[f0 + lane];
+                    s_a[lane] = Avalx[xc + lane];
+                    ce = Browp[k + 1];
+                    cb = part == p0 ? lower_bound_i(Bcolx, Browp[k], ce, lo) : cur[xc - xa + lane];
+                    int col[HEAVY_STAGE];
+                    double val[HEAVY_STAGE];
+#pragma unroll
+                    for (int j = 0; j < HEAVY_STAGE; j++) {
+                        col[j] = cb + j < ce ? Bcolx[cb + j] : 0x7fffffff;
+                        val[j] = cb + j < ce ? Bvalx[cb + j] : 0.0;
                     }
-                    const bool in = col < hi;  // in-range entries form a prefix
-                    if (in) s_acc[col - lo] = __builtin_fma(a, bv, s_acc[col - lo]);
-                    int e = e0 + __popcll(__ballot(in));
-                    if (e == e0 + 64) {  // more than 64 entries in the part
-                        while (e < e1) {
-                            const int ee = e + lane;
-                            const int c = ee < e1 ? Bcolx[ee] : 0x7fffffff;
-                            const bool in2 = c < hi;
-                            if (in2) s_acc[c - lo] = __builtin_fma(a, Bvalx[ee], s_acc[c - lo]);
-                            const int cnt = __popcll(__ballot(in2));
-                            e += cnt;
-                            if (cnt < 64) break;
+#pragma unroll
+                    for (int j = 0; j < HEAVY_STAGE; j++) {
+                        const bool in = col[j] < hi;  // in-range entries form a prefix
+                        cnt += in;
+                        if (in) {
+                            s_sc[lane * HEAVY_STAGE + j] = col[j];
+                            s_sv[lane * HEAVY_STAGE + j] = val[j];
                         }
                     }
-                    if (lane == q) next = e;
+                    more = cnt == HEAVY_STAGE && cb + HEAVY_STAGE < ce;
+                    s_n[lane] = cnt | (more ? 256 : 0);
+                }
+                int next = cb + cnt;  // lane q: where source q's next part starts
+                tsg_wave_sync();
+                // the staged data of source q+1 is read before source q's
+                // accumulation, which then costs one LDS round trip
+                int sn = s_n[0], sc = 0;
+                double a = s_a[0], sv = 0.0;
+                if (lane < (sn & 255)) {
+                    sc = s_sc[lane];
+                    sv = s_sv[lane];
+                }
+                for (int q = 0; q < n; q++) {  // ascending k; entries of one k have distinct columns
+                    int sn1 = 0, sc1 = 0;
+                    double a1 = 0.0, sv1 = 0.0;
+                    if (q + 1 < n) {
+                        sn1 = s_n[q + 1];
+                        a1 = s_a[q + 1];
+                        if (lane < (sn1 & 255)) {
+                            sc1 = s_sc[(q + 1) * HEAVY_STAGE + lane];
+                            sv1 = s_sv[(q + 1) * HEAVY_STAGE + lane];
+                        }
+                    }
+                    if (lane < (sn & 255)) s_acc[sc - lo] = __builtin_fma(a, sv, s_acc[sc - lo]);
+                    if (sn & 256) {  // entries past the staged ones, 256 per round trip
+                        int e = __shfl(cb, q, 64) + HEAVY_STAGE;
+                        const int e1 = __shfl(ce, q, 64);
+                        while (e < e1) {
+                            int c[4];
+                            double v[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int ee = e + 64 * u + lane;
+                                c[u] = ee < e1 ? Bcolx[ee] : 0x7fffffff;
+                                v[u] = ee < e1 ? Bvalx[ee] : 0.0;
+                            }
+                            int got = 0;
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const bool in2 = c[u] < hi;
+                                if (in2) s_acc[c[u] - lo] = __builtin_fma(a, v[u], s_acc[c[u] - lo]);
+                                got += __popcll(__ballot(in2));
+                            }
+                            e += got;
+                            if (got < 256) break;
+                        }
+                        if (lane == q) next = e;
+                    }
                     tsg_wave_sync();  // the next k reads what this one wrote
-                    col = ncol;
-                    bv = nbv;
-                    e0 = f0;
-                    e1 = f1;
+                    sn = sn1;
+                    sc = sc1;
+                    a = a1;
+                    sv = sv1;
                 }
                 if (part + 1 < p1 && lane < n) cur[xc - xa + lane] = next;
             }
