@@ -1375,7 +1375,6 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
     const int nbase = Cnnz[c0], nrow = Cnnz[c0 + u] - nbase;
     const int a0 = op.Aptr[I], la = op.Aptr[I + 1] - a0;
-    for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
     for (int i = tid; i < u; i += G3_THREADS) {
         s_ccol[i] = Ccol[c0 + i];
         s_coff[i] = Cnnz[c0 + i] - nbase;
@@ -1405,20 +1404,37 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (lane == 63) s_npair = incl;
     }
     __syncthreads();
-    // C tile slot of every (A tile, B tile) pair of the tile-row
+    // C tile slot of every (A tile, B tile) pair of the tile-row.  When the
+    // row's C tile columns span at most the accumulator's bytes, the
+    // accumulator area first holds a direct map column -> slot (every pair's
+    // column is in the row's C tile list, so unmapped bytes are never read);
+    // otherwise a binary search over the sorted C tile columns.
+    const int jlo = s_ccol[0], jspan = s_ccol[u - 1] - jlo + 1;
+    const bool direct = jspan <= 8 * L.nnzmax;
+    unsigned char *s_map = reinterpret_cast<unsigned char *>(s_acc);
+    if (direct) {
+        for (int i = tid; i < u; i += G3_THREADS) s_map[s_ccol[i] - jlo] = (unsigned char)i;
+        __syncthreads();
+    }
     for (int x = tid; x < s_npair; x += G3_THREADS) {
         int j = 0;
 #pragma unroll
         for (int st = G3_LAMAX / 2; st > 0; st >>= 1)
             if (j + st < la && SROW(7, j + st) <= x) j += st;
         const int J = op.Bcol[SROW(3, j) + (x - SROW(7, j))];
-        int lo = 0, hi = u - 1;  // J is in the step-1 list
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+        if (direct) {
+            s_pslot[x] = s_map[J - jlo];
+        } else {
+            int lo = 0, hi = u - 1;  // J is in the step-1 list
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+            }
+            s_pslot[x] = (unsigned char)lo;
         }
-        s_pslot[x] = (unsigned char)lo;
     }
+    __syncthreads();  // the map area becomes the accumulator
+    for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
     G3_SLOT_DECL(R0_);
     G3_SLOT_DECL(R1_);
     G3_SLOT_DECL(R2_);
