@@ -339,12 +339,18 @@ typedef struct {
     int fused_with;           /* requests in the same fused product (1 = alone) */
     long long numtileC, nnzC, bytes;
     double t_arrival_ms, t_admit_ms, t_done_ms; /* server clock */
+    double t_gpu_ms;          /* device time of its pipeline (of the fused
+                                 product when fused) */
 } tsg_request_info;
 
 typedef struct {
     long long resident_bytes, reserved_bytes, peak_reserved_bytes;
     int queued;
     long long fused_launches, single_launches;
+    long long cached_bytes;         /* free blocks held by the workers' allocators */
+    long long peak_footprint_bytes; /* most of resident + reserved + cached, which
+                                       bounds the server's device memory and stays
+                                       within the budget */
 } tsg_server_stats_t;
 
 int tsg_server_create(int device, const tsg_server_config *cfg, tsg_server **out);

@@ -56,12 +56,18 @@ extern "C" int tsg_ctx_create(int device, tsg_ctx **out) {
     return TSG_OK;
 }
 
+// A block goes back to the arena it came from, else to the driver.
+static void release_block(tsg_ctx *c, void *p, size_t sz) {
+    if (tsg_arena_owns(c->arena, p)) tsg_arena_free(c->arena, p, sz);
+    else hipFree(p);
+}
+
 extern "C" void tsg_ctx_destroy(tsg_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    for (auto &kv : c->free_blocks) hipFree(kv.second);
-    for (auto &kv : c->live) hipFree(kv.first);
+    for (auto &kv : c->free_blocks) release_block(c, kv.second, kv.first);
+    for (auto &kv : c->live) release_block(c, kv.first, kv.second);
     if (c->scratch) hipFree(c->scratch);
     if (c->pinned) hipHostFree(c->pinned);
     hipStreamDestroy(c->stream);
@@ -78,10 +84,88 @@ extern "C" long long tsg_ctx_bytes_in_use(tsg_ctx *c) { return c->bytes_live; }
 extern "C" long long tsg_ctx_bytes_cached(tsg_ctx *c) { return c->bytes_cached; }
 
 extern "C" void tsg_ctx_trim(tsg_ctx *c) {
-    hipStreamSynchronize(c->stream);
-    for (auto &kv : c->free_blocks) hipFree(kv.second);
+    hipStreamSynchronize(c->stream);  // no pending kernel still uses a cached block
+    for (auto &kv : c->free_blocks) release_block(c, kv.second, kv.first);
     c->free_blocks.clear();
     c->bytes_cached = 0;
+}
+
+void tsg_ctx_set_arena(tsg_ctx *c, tsg_arena *a) { c->arena = a; }
+
+// ------------------------------------------------------------------ arena
+int tsg_arena_create(int device, size_t bytes, tsg_arena **out) {
+    *out = nullptr;
+    bytes = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+    TSG_HIP(hipSetDevice(device));
+    tsg_arena *a = new tsg_arena();
+    a->device = device;
+    hipError_t e = hipMalloc((void **)&a->base, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        tsg_set_error("arena: device allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+        delete a;
+        return TSG_ERR_NOMEM;
+    }
+    a->size = bytes;
+    a->free_by_off[0] = bytes;
+    a->free_by_len.emplace(bytes, 0);
+    a->bytes_free = bytes;
+    *out = a;
+    return TSG_OK;
+}
+
+void tsg_arena_destroy(tsg_arena *a) {
+    if (!a) return;
+    hipSetDevice(a->device);
+    hipFree(a->base);
+    delete a;
+}
+
+static void arena_erase_len(tsg_arena *a, size_t len, size_t off) {
+    auto r = a->free_by_len.equal_range(len);
+    for (auto it = r.first; it != r.second; ++it)
+        if (it->second == off) {
+            a->free_by_len.erase(it);
+            return;
+        }
+}
+
+void *tsg_arena_alloc(tsg_arena *a, size_t bytes) {
+    std::lock_guard<std::mutex> g(a->mu);
+    auto it = a->free_by_len.lower_bound(bytes);
+    if (it == a->free_by_len.end()) return nullptr;
+    const size_t len = it->first, off = it->second;
+    a->free_by_len.erase(it);
+    a->free_by_off.erase(off);
+    if (len > bytes) {
+        a->free_by_off[off + bytes] = len - bytes;
+        a->free_by_len.emplace(len - bytes, off + bytes);
+    }
+    a->bytes_free -= bytes;
+    return a->base + off;
+}
+
+void tsg_arena_free(tsg_arena *a, void *p, size_t bytes) {
+    std::lock_guard<std::mutex> g(a->mu);
+    size_t off = (size_t)((char *)p - a->base), len = bytes;
+    a->bytes_free += bytes;
+    auto nx = a->free_by_off.lower_bound(off);
+    if (nx != a->free_by_off.end() && nx->first == off + len) {  // merge the next piece
+        arena_erase_len(a, nx->second, nx->first);
+        len += nx->second;
+        nx = a->free_by_off.erase(nx);
+    }
+    if (nx != a->free_by_off.begin()) {  // merge the previous piece
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            arena_erase_len(a, pv->second, pv->first);
+            off = pv->first;
+            len += pv->second;
+            a->free_by_off.erase(pv);
+        }
+    }
+    a->free_by_off[off] = len;
+    a->free_by_len.emplace(len, off);
 }
 
 // Allocation sizes are rounded to 512 B (small) or 2 MiB (large) classes;
@@ -107,13 +191,23 @@ int tsg_alloc(tsg_ctx *c, void **p, size_t bytes) {
         c->free_blocks.erase(it);
         return TSG_OK;
     }
+    if (c->arena) {
+        *p = tsg_arena_alloc(c->arena, sz);
+        if (!*p && !c->free_blocks.empty()) {  // give the cache back and retry
+            tsg_ctx_trim(c);
+            *p = tsg_arena_alloc(c->arena, sz);
+        }
+        if (*p) {
+            c->live[*p] = sz;
+            c->bytes_live += sz;
+            return TSG_OK;
+        }
+        // the arena is full: fall back to the driver
+    }
     hipError_t e = hipMalloc(p, sz);
     if (e != hipSuccess) {
         // release the cache and retry once
-        hipStreamSynchronize(c->stream);
-        for (auto &kv : c->free_blocks) hipFree(kv.second);
-        c->free_blocks.clear();
-        c->bytes_cached = 0;
+        tsg_ctx_trim(c);
         (void)hipGetLastError();
         e = hipMalloc(p, sz);
         if (e != hipSuccess) {
@@ -127,6 +221,11 @@ int tsg_alloc(tsg_ctx *c, void **p, size_t bytes) {
     c->live[*p] = sz;
     c->bytes_live += sz;
     return TSG_OK;
+}
+
+size_t tsg_alloc_size(const tsg_ctx *c, const void *p) {
+    auto it = c->live.find(const_cast<void *>(p));
+    return it == c->live.end() ? 0 : it->second;
 }
 
 void tsg_free(tsg_ctx *c, void *p) {

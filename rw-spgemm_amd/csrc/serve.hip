@@ -51,6 +51,16 @@ long long c_bytes(long long tilem, long long numtile, long long nnz, int s1w) {
     return arrays + (long long)tsg_spgemm_work_bytes(&c, s1w);
 }
 
+// Device bytes a tile matrix allocated on ctx holds.
+long long held_bytes(const tsg_ctx *ctx, const tsg_dtiles &t) {
+    const void *arr[] = {t.tile_ptr, t.tile_colidx, t.tile_rowidx, t.tile_nnz, t.csr_ptr, t.csr_col, t.val,
+                         t.mask, t.csc_ptr, t.csc_rowidx, t.csc_tileid, t.rowx_ptr, t.rowx_col, t.rowx_val};
+    long long b = 0;
+    for (const void *p : arr)
+        if (p) b += (long long)tsg_alloc_size(ctx, p);
+    return b;
+}
+
 long long tile_bytes(const tsg_dtiles &t) {
     long long b = 4LL * (t.tilem + 1) + 12LL * (t.numtile + 1) + 48LL * t.numtile + 9LL * t.nnz;
     if (t.csc_ptr) b += 4LL * (t.tilen + 1) + 8LL * t.numtile;
@@ -78,6 +88,7 @@ struct Request {
     unsigned long long id;
     int handle, op;
     double t_arrival = 0, t_admit = 0, t_done = 0;
+    double t_gpu = 0;  // device time of its pipeline runs
     int status = TSG_OK;
     long long bytes = 0;
     bool done = false, released = false, auto_release = false;
@@ -116,6 +127,15 @@ struct tsg_server {
     long long fused_launches = 0, single_launches = 0;
     // device memory to free, per worker (allocators are per context)
     std::vector<std::vector<tsg_dtiles>> pending_free;
+    // free blocks each worker's allocator holds, and trim requests: cached
+    // blocks count against the budget, so that resident + reserved + cached
+    // bounds the device memory the server uses
+    std::vector<long long> wcached;
+    std::vector<char> trim_req;
+    long long peak_footprint = 0;
+    // every context allocates from one arena of the budget's size, reserved
+    // at creation: trimming a cache returns blocks to it, never to the driver
+    tsg_arena *arena = nullptr;
 };
 
 extern "C" void tsg_server_config_default(tsg_server_config *c) {
@@ -139,17 +159,28 @@ extern "C" int tsg_server_create(int device, const tsg_server_config *cfg, tsg_s
     if (s->cfg.max_batch < 1) s->cfg.max_batch = 1;
     if (s->cfg.chunk_products < 1) s->cfg.chunk_products = 1500000000LL;
     s->device = device;
-    int st = tsg_ctx_create(device, &s->main);
+    int st = tsg_arena_create(device, (size_t)std::max(1LL, s->cfg.budget_bytes), &s->arena);
     if (st) { delete s; return st; }
+    st = tsg_ctx_create(device, &s->main);
+    if (st) {
+        tsg_arena_destroy(s->arena);
+        delete s;
+        return st;
+    }
+    tsg_ctx_set_arena(s->main, s->arena);
     s->wctx.resize(s->cfg.slots, nullptr);
     s->pending_free.resize(s->cfg.slots);
+    s->wcached.assign(s->cfg.slots, 0);
+    s->trim_req.assign(s->cfg.slots, 0);
     for (int w = 0; w < s->cfg.slots; w++) {
         st = tsg_ctx_create(device, &s->wctx[w]);
         if (st) break;
+        tsg_ctx_set_arena(s->wctx[w], s->arena);
     }
     if (st) {
         for (auto *c : s->wctx) if (c) tsg_ctx_destroy(c);
         tsg_ctx_destroy(s->main);
+        tsg_arena_destroy(s->arena);
         delete s;
         return st;
     }
@@ -279,11 +310,39 @@ extern "C" int tsg_server_submit(tsg_server *s, int handle, int op, unsigned lon
     return submit_locked(s, handle, op, false, id);
 }
 
-// Picks the next job under s->mu: returns the admitted request ids.
-static std::vector<Request *> pick(tsg_server *s) {
+// Under s->mu: bytes cached by the workers other than w.
+static long long others_cached(const tsg_server *s, int w) {
+    long long c = 0;
+    for (int v = 0; v < (int)s->wcached.size(); v++)
+        if (v != w) c += s->wcached[v];
+    return c;
+}
+
+// Caches with a trim pending are about to be freed before their worker's
+// next job allocates anything.
+static void note_footprint(tsg_server *s) {
+    long long c = 0;
+    for (int v = 0; v < (int)s->wcached.size(); v++)
+        if (!s->trim_req[v]) c += s->wcached[v];
+    s->peak_footprint = std::max(s->peak_footprint, s->resident + s->reserved + c);
+}
+
+// Picks the next job for worker w under s->mu: returns the admitted
+// requests.  Worker w's own cached blocks are room for its job (it trims
+// them if the job does not fit beside them); the other workers' count as
+// used.  When only those caches keep the queue head out, their workers are
+// asked to trim.
+static std::vector<Request *> pick(tsg_server *s, int w) {
     std::vector<Request *> job;
-    const long long avail = s->cfg.budget_bytes - s->resident - s->reserved;
+    const long long free_budget = s->cfg.budget_bytes - s->resident - s->reserved;
+    const long long avail = free_budget - others_cached(s, w);
     const double now = clock_ms();
+    if (!s->queue.empty()) {
+        const Request *head = s->reqs[s->queue[0]].get();
+        if (head->bytes > avail && head->bytes <= free_budget)
+            for (int v = 0; v < (int)s->wcached.size(); v++)
+                if (v != w && s->wcached[v] > 0) s->trim_req[v] = 1;
+    }
     size_t qi = 0;
     for (; qi < s->queue.size(); qi++) {
         Request *r = s->reqs[s->queue[qi]].get();
@@ -309,17 +368,23 @@ static std::vector<Request *> pick(tsg_server *s) {
         s->reserved += r->bytes;
     }
     s->peak_reserved = std::max(s->peak_reserved, s->reserved);
+    // the worker's cache beside the job must fit what is left; else it is
+    // trimmed before the job runs (other workers count it until then)
+    if (s->wcached[w] > s->cfg.budget_bytes - s->resident - s->reserved - others_cached(s, w)) s->trim_req[w] = 1;
+    note_footprint(s);
     return job;
 }
 
 static void free_request_results(tsg_server *s, Request *r) {
     // under s->mu: hand device memory to its worker
+    // (the blocks reach the worker's cache when it frees them: counted there now)
     for (auto &c : r->chunks) s->pending_free[r->worker].push_back(c);
     r->chunks.clear();
     if (r->fused) {
         if (--r->fused->live == 0) s->pending_free[r->fused->worker].push_back(r->fused->C);
         r->fused.reset();
     }
+    s->wcached[r->worker] += r->bytes;
     s->reserved -= r->bytes;
     r->bytes = 0;
 }
@@ -330,7 +395,9 @@ static int run_single(tsg_server *s, tsg_ctx *ctx, Request *r) {
     const OpSize &o = M.size[r->op];
     for (size_t c = 0; c + 1 < o.bounds.size(); c++) {
         tsg_dtiles C;
-        TSG_TRY(tsg_spgemm_tile_rows(ctx, &M.ta, &B, o.bounds[c], o.bounds[c + 1], &s->cfg.tune, &C, nullptr));
+        tsg_stats st;
+        TSG_TRY(tsg_spgemm_tile_rows(ctx, &M.ta, &B, o.bounds[c], o.bounds[c + 1], &s->cfg.tune, &C, &st));
+        r->t_gpu += st.t_total_ms;
         r->chunks.push_back(C);
         r->chunk_r0.push_back(o.bounds[c]);
     }
@@ -351,7 +418,10 @@ static int run_fused(tsg_server *s, tsg_ctx *ctx, std::vector<Request *> &job, s
     if (st) { tsg_tiles_free(ctx, &A); return st; }
     f = std::make_shared<Fused>();
     f->live = n;
-    st = tsg_spgemm(ctx, &A, &B, &s->cfg.tune, &f->C, nullptr);
+    tsg_stats ss;
+    st = tsg_spgemm(ctx, &A, &B, &s->cfg.tune, &f->C, &ss);
+    if (!st)
+        for (Request *r : job) r->t_gpu = ss.t_total_ms;
     if (!st) st = tsg_ctx_sync(ctx);
     tsg_tiles_free(ctx, &A);
     tsg_tiles_free(ctx, &B);
@@ -372,24 +442,38 @@ static void worker_main(tsg_server *s, int w) {
     for (;;) {
         std::vector<Request *> job;
         std::vector<tsg_dtiles> to_free;
+        bool trim = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
                 to_free.swap(s->pending_free[w]);
-                if (!to_free.empty() || s->stop) break;
-                job = pick(s);
-                if (!job.empty()) break;
+                if (s->trim_req[w]) {
+                    trim = true;
+                    s->trim_req[w] = 0;
+                }
+                if (trim || !to_free.empty() || s->stop) break;
+                job = pick(s, w);
+                if (!job.empty()) {
+                    trim = s->trim_req[w];  // the job needs this worker's cache trimmed first
+                    break;
+                }
+                if (std::find(s->trim_req.begin(), s->trim_req.end(), 1) != s->trim_req.end())
+                    s->cv_work.notify_all();  // other workers were asked to trim
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
             }
-            if (job.empty() && to_free.empty() && s->stop) return;
+            if (job.empty() && to_free.empty() && !trim && s->stop) return;
         }
         for (auto &t : to_free) tsg_tiles_free(ctx, &t);
-        // cached free blocks count against the device, not the budget:
-        // return them when they pile up
-        if (!to_free.empty() && tsg_ctx_bytes_cached(ctx) > s->cfg.budget_bytes / (2 * s->cfg.slots))
+        if (trim) {
             tsg_ctx_trim(ctx);
+            std::lock_guard<std::mutex> g(s->mu);
+            s->wcached[w] = tsg_ctx_bytes_cached(ctx);
+            s->trim_req[w] = 0;
+        }
         if (job.empty()) {
             std::lock_guard<std::mutex> g(s->mu);
+            s->wcached[w] = tsg_ctx_bytes_cached(ctx);
+            note_footprint(s);
             s->cv_work.notify_all();  // memory came back
             continue;
         }
@@ -401,7 +485,25 @@ static void worker_main(tsg_server *s, int w) {
             st = run_fused(s, ctx, job, f);
             if (f) f->worker = w;
         }
-        std::lock_guard<std::mutex> g(s->mu);
+        long long held = -1;  // a single run's result: its transient blocks are now cached
+        if (!st && job.size() == 1) {
+            held = 0;
+            for (auto &c : job[0]->chunks) held += held_bytes(ctx, c);
+        }
+        std::unique_lock<std::mutex> g(s->mu);
+        if (held >= 0 && held < job[0]->bytes) {
+            s->reserved -= job[0]->bytes - held;
+            job[0]->bytes = held;
+        }
+        s->wcached[w] = tsg_ctx_bytes_cached(ctx);
+        if (s->resident + s->reserved + others_cached(s, w) + s->wcached[w] > s->cfg.budget_bytes) {
+            // the cache beside what is held would exceed the budget
+            g.unlock();
+            tsg_ctx_trim(ctx);
+            g.lock();
+            s->wcached[w] = tsg_ctx_bytes_cached(ctx);
+        }
+        note_footprint(s);
         const double now = clock_ms();
         if (job.size() == 1) s->single_launches++; else s->fused_launches++;
         for (Request *r : job) {
@@ -413,6 +515,7 @@ static void worker_main(tsg_server *s, int w) {
             if (st) {  // no result: give back its memory now
                 for (auto &c : r->chunks) s->pending_free[w].push_back(c);
                 r->chunks.clear();
+                s->wcached[w] += r->bytes;
                 s->reserved -= r->bytes;
                 r->bytes = 0;
                 r->released = true;
@@ -457,6 +560,7 @@ extern "C" int tsg_server_wait(tsg_server *s, unsigned long long id, double time
         info->t_arrival_ms = r->t_arrival;
         info->t_admit_ms = r->t_admit;
         info->t_done_ms = r->t_done;
+        info->t_gpu_ms = r->t_gpu;
     }
     return r->status;
 }
@@ -520,6 +624,7 @@ extern "C" int tsg_server_result_csr(tsg_server *s, unsigned long long id, tsg_h
         TSG_TRY(st);
         nnz += parts[i].nnz;
     }
+    tsg_ctx_trim(ctx);  // the CSR staging goes back to the arena
     out->m = m;
     out->n = n;
     out->nnz = (int)nnz;
@@ -554,6 +659,8 @@ extern "C" int tsg_server_stats(tsg_server *s, tsg_server_stats_t *out) {
     out->queued = (int)s->queue.size();
     out->fused_launches = s->fused_launches;
     out->single_launches = s->single_launches;
+    for (long long c : s->wcached) out->cached_bytes += c;
+    out->peak_footprint_bytes = s->peak_footprint;
     return TSG_OK;
 }
 
@@ -613,5 +720,6 @@ extern "C" void tsg_server_destroy(tsg_server *s) {
         tsg_dcsr_free(s->main, &M->a);
     }
     tsg_ctx_destroy(s->main);
+    tsg_arena_destroy(s->arena);
     delete s;
 }

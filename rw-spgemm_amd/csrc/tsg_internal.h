@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,28 @@ void tsg_set_error(const char *fmt, ...);
         }                                                                      \
     } while (0)
 
+// ------------------------------------------------------------------ arena
+// One device allocation handed out in pieces (best fit; released pieces
+// coalesce with free neighbours); shared by contexts on several threads.
+// Memory returned to an arena stays mapped, so a workload that keeps
+// allocating and releasing large blocks does not go back to the driver.
+struct tsg_arena {
+    int device = 0;
+    char *base = nullptr;
+    size_t size = 0;
+    std::mutex mu;
+    std::map<size_t, size_t> free_by_off;       // offset -> length
+    std::multimap<size_t, size_t> free_by_len;  // length -> offset
+    size_t bytes_free = 0;
+};
+int tsg_arena_create(int device, size_t bytes, tsg_arena **out);
+void tsg_arena_destroy(tsg_arena *a);
+void *tsg_arena_alloc(tsg_arena *a, size_t bytes);  // nullptr when no free piece fits
+void tsg_arena_free(tsg_arena *a, void *p, size_t bytes);
+inline bool tsg_arena_owns(const tsg_arena *a, const void *p) {
+    return a && (const char *)p >= a->base && (const char *)p < a->base + a->size;
+}
+
 // ------------------------------------------------------------------ context
 struct tsg_ctx {
     int device;
@@ -58,7 +81,11 @@ struct tsg_ctx {
     size_t scratch_bytes = 0;
     int num_cu = 0;
     size_t lds_per_block = 0;
+    // when set, blocks come from this arena (hipMalloc only if it is full)
+    tsg_arena *arena = nullptr;
 };
+// Routes ctx's allocations through an arena; call before it allocates.
+void tsg_ctx_set_arena(tsg_ctx *ctx, tsg_arena *a);
 
 int tsg_alloc(tsg_ctx *ctx, void **p, size_t bytes);
 void tsg_free(tsg_ctx *ctx, void *p);
@@ -68,6 +95,8 @@ static inline int tsg_alloc_t(tsg_ctx *ctx, T **p, size_t count) {
 }
 int tsg_scratch(tsg_ctx *ctx, size_t bytes, void **p);
 int tsg_read_scalar_i64(tsg_ctx *ctx, const void *dptr, int is64, long long *out);
+// Bytes the allocator reserved for a live block of ctx (0 if none).
+size_t tsg_alloc_size(const tsg_ctx *ctx, const void *p);
 
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n non-negative int32 counts into out[0..n] (out[n] =

@@ -123,7 +123,7 @@ class ServerConfig(C.Structure):
 class RequestInfo(C.Structure):
     _fields_ = [("status", _i), ("handle", _i), ("op", _i), ("chunks", _i), ("fused_with", _i),
                 ("numtileC", _ll), ("nnzC", _ll), ("bytes", _ll), ("t_arrival_ms", _d), ("t_admit_ms", _d),
-                ("t_done_ms", _d)]
+                ("t_done_ms", _d), ("t_gpu_ms", _d)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -131,7 +131,8 @@ class RequestInfo(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("resident_bytes", _ll), ("reserved_bytes", _ll), ("peak_reserved_bytes", _ll), ("queued", _i),
-                ("fused_launches", _ll), ("single_launches", _ll)]
+                ("fused_launches", _ll), ("single_launches", _ll), ("cached_bytes", _ll),
+                ("peak_footprint_bytes", _ll)]
 
 
 def _sig(name, res, args):

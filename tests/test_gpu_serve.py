@@ -117,6 +117,8 @@ def test_serve_budget(ctx):
         ids, infos = srv.replay([ha] * 12 + [hb] * 4, [ts.OP_A2] * 16, [0.0] * 16, keep=False)
         assert all(i.status == 0 for i in infos)
         assert srv.stats().peak_reserved_bytes + st.resident_bytes <= 256 << 20
+        # with the workers' cached blocks counted too
+        assert srv.stats().peak_footprint_bytes <= 256 << 20
         big = ts.gen_er(60000, 30.0, seed=7)
         with pytest.raises(ts.TsgError):
             hbig = srv.register(big)
