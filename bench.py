@@ -61,10 +61,49 @@ def cpu_baseline(a, nnzcub_per_row, target_s=15.0):
         tt, _ = oracle.spa_structure_rows_timed(a, a, r0, r1)
         times.append(tt)
     tmin = min(times)
+    # the value-computing SPA (src/external/cusparse/spgemm_serialref_spa.h
+    # order, touched-entry reset) on the same rows
+    vt = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        oracle.spgemm_rows(a, a, r0, r1, values=True)
+        vt.append(time.perf_counter() - t0)
     return {"value": round(2.0 * cub / tmin / 1e9, 4), "unit": "GFLOPS", "cores": oracle.num_threads(),
             "kind": "port",
             "sample": f"rows [{r0},{r1}) of the same A^2 ({r1 - r0} of {a.m} rows, {cub} products), "
-                      f"count+fill passes, min of {len(times)} runs ({tmin:.2f} s)"}
+                      f"count+fill passes, min of {len(times)} runs ({tmin:.2f} s)",
+            "values_spa": {"value": round(2.0 * cub / min(vt) / 1e9, 4), "unit": "GFLOPS",
+                           "sample": f"same rows, structure + FP64 values, min of {len(vt)} runs ({min(vt):.2f} s)"}}
+
+
+def dirichlet_leg(ts, ctx, N, tune, warmup, steps):
+    """The same pipeline on the 27-point Dirichlet stencil N^3 (rows at the
+    boundary keep only in-grid neighbours); the headline is the periodic
+    stencil."""
+    a = ts.gen_stencil27(N, N, N, periodic=False, valmode=ts.VAL_UNIFORM_PM1, seed=1)
+    da = ctx.upload(a)
+    ta = ctx.csr2tile(da)
+    ctx.build_csc(ta)
+    cub = ctx.nnzcub(da, da)
+    for _ in range(warmup):
+        tc, _ = ctx.spgemm(ta, ta, tune)
+        ctx.free(tc)
+    ctx.sync()
+    ev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tc, st = ctx.spgemm(ta, ta, tune)
+        ev.append(st.t_total_ms)
+        ctx.free(tc)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": f"27-point Dirichlet stencil {N}^3, C = A^2", "n": a.m, "nnz": a.nnz, "nnzCub": cub,
+           "nnzC": st.nnzC, "tilesC": st.numtileC, "gflops": round(2.0 * cub / (ms * 1e6), 2),
+           "ms_per_step": round(ms, 4), "gflops_event_min": round(2.0 * cub / (min(ev) * 1e6), 2)}
+    ctx.free(ta)
+    ctx.free(da)
+    ctx.trim()
+    return out
 
 
 def progress(msg):
@@ -355,6 +394,7 @@ def main():
     ap.add_argument("--element-mode", type=int, default=None,
                     help="build B's row index and use this element_mode (0 auto, 1 never, 2 every tile-row)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-27d", action="store_true")
     ap.add_argument("--rmat-only", action="store_true")
     ap.add_argument("--rmat-scale", type=int, default=20)
     ap.add_argument("--serve-only", action="store_true")
@@ -453,6 +493,8 @@ def main():
     }
     ctx.free(ta)
     ctx.free(da)
+    if not args.no_27d:
+        out["stencil27_dirichlet"] = dirichlet_leg(ts, ctx, N, tune, args.warmup, args.steps)
     if not args.no_batch:
         ctx.trim()
         out["batched"] = batch_leg(ts, ctx, count=args.batch_count, autotune=not args.no_autotune)
