@@ -817,6 +817,7 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
 // stored + 1 (0 = empty), value = the entry's position in the segment.
 #define G3E_EHASH 4096  // >= 2 * G3E_NNZ
+#define G3E_PF 8         // C tiles per thread whose metadata is loaded per round
 __device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 2654435761u) >> 20; }  // 12 bits
 
 __global__ void __launch_bounds__(256)
@@ -828,6 +829,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     __shared__ double s_acc[G3E_NNZ];
     __shared__ unsigned s_ekey[G3E_EHASH];
     __shared__ unsigned short s_epos[G3E_EHASH];
+    __shared__ unsigned char s_ecol[G3E_NNZ];  // in-tile column of each entry, final order
     const int tid = threadIdx.x;
     // the grid is an upper bound: the first *nseg blocks (spread evenly over
     // the XCDs) take the segments, each XCD a contiguous range
@@ -842,25 +844,45 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     for (int i = tid; i < nseg_nnz; i += 256) s_acc[i] = 0.0;
     for (int i = tid; i < G3E_EHASH; i += 256) s_ekey[i] = 0u;
     __syncthreads();
-    // every C entry of the segment into the hash (one C tile per thread;
-    // structurally empty tiles skipped)
-    for (int i = tid; i < u; i += 256) {
-        const long long t = c0 + i;
-        int pos = Cnnz[t] - nbase;
-        if (Cnnz[t + 1] - Cnnz[t] == 0) continue;
-        const uint4 *cm4 = reinterpret_cast<const uint4 *>(Cmask + t * 16);
-        const uint4 mA = cm4[0], mB = cm4[1];
-        const unsigned mw[8] = {mA.x, mA.y, mA.z, mA.w, mB.x, mB.y, mB.z, mB.w};
-        const unsigned j16 = (unsigned)(Ccol[t] * 16 - col_lo);
-        for (int rr = 0; rr < 16; rr++) {  // rows in order: positions follow
-            unsigned cm = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
-            while (cm) {
-                const int bb = 31 - __clz(cm);
-                cm &= ~(1u << bb);
-                const unsigned key = (((j16 + 15 - bb) << 4) | (unsigned)rr) + 1u;
-                unsigned h = ent_hash(key);
-                while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
-                s_epos[h] = (unsigned short)pos++;
+    // every C entry of the segment into the hash, and its in-tile column
+    // into s_ecol (one C tile per thread; structurally empty tiles skipped).
+    // The metadata of up to G3E_PF of a thread's tiles is loaded in one
+    // round before their entries are inserted.
+    for (int i0 = 0; i0 < u; i0 += 256 * G3E_PF) {
+        int pos0[G3E_PF], nt[G3E_PF];
+        unsigned j16[G3E_PF];
+        uint4 mA[G3E_PF], mB[G3E_PF];
+#pragma unroll
+        for (int v = 0; v < G3E_PF; v++) {
+            const int i = i0 + v * 256 + tid;
+            nt[v] = 0;
+            if (i < u) {
+                const long long t = c0 + i;
+                pos0[v] = Cnnz[t] - nbase;
+                nt[v] = Cnnz[t + 1] - Cnnz[t];
+                const uint4 *cm4 = reinterpret_cast<const uint4 *>(Cmask + t * 16);
+                mA[v] = cm4[0];
+                mB[v] = cm4[1];
+                j16[v] = (unsigned)(Ccol[t] * 16 - col_lo);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < G3E_PF; v++) {
+            if (nt[v] == 0) continue;
+            const unsigned mw[8] = {mA[v].x, mA[v].y, mA[v].z, mA[v].w, mB[v].x, mB[v].y, mB[v].z, mB[v].w};
+            int pos = pos0[v];
+            for (int rr = 0; rr < 16; rr++) {  // rows in order: positions follow
+                unsigned cm = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
+                while (cm) {
+                    const int bb = 31 - __clz(cm);
+                    cm &= ~(1u << bb);
+                    const unsigned key = (((j16[v] + 15 - bb) << 4) | (unsigned)rr) + 1u;
+                    unsigned h = ent_hash(key);
+                    while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
+                    s_epos[h] = (unsigned short)pos;
+                    s_ecol[pos] = (unsigned char)(15 - bb);
+                    pos++;
+                }
             }
         }
     }
@@ -992,22 +1014,9 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
         }
     }
     __syncthreads();
-    for (int i = tid; i < nseg_nnz; i += 256) Cval[(long long)nbase + i] = s_acc[i];
-    for (int i = tid; i < u; i += 256) {
-        const long long t = c0 + i;
-        if (Cnnz[t + 1] - Cnnz[t] == 0) continue;
-        const uint4 *cm4 = reinterpret_cast<const uint4 *>(Cmask + t * 16);
-        const uint4 mA = cm4[0], mB = cm4[1];
-        const unsigned mw[8] = {mA.x, mA.y, mA.z, mA.w, mB.x, mB.y, mB.z, mB.w};
-        unsigned char *oc = Ccol8 + Cnnz[t];
-        for (int rr = 0; rr < 16; rr++) {
-            unsigned cm = (mw[rr >> 1] >> ((rr & 1) * 16)) & 0xffffu;
-            while (cm) {
-                const int bb = 31 - __clz(cm);
-                cm &= ~(1u << bb);
-                *oc++ = (unsigned char)(15 - bb);
-            }
-        }
+    for (int i = tid; i < nseg_nnz; i += 256) {
+        Cval[(long long)nbase + i] = s_acc[i];
+        Ccol8[(long long)nbase + i] = s_ecol[i];
     }
 }
 
