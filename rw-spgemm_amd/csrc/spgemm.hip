@@ -261,7 +261,7 @@ __device__ __forceinline__ int seg_walk(const int *__restrict__ Cnnz, int c0, in
                 }
                 s1 = lo;
             }
-            out[j] = make_int4(I, s0, s1, 0);
+            out[j] = make_int4(I, s0, s1, j);  // .w: index of the segment in its tile-row
             s0 = s1;
         }
     }
@@ -536,7 +536,9 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
             const int e1 = Browp[k + 1];
             bool more = true;
             // B row k's entries in the segment's column range, 4 loads at a time
-            for (int e = lower_bound_i(Bcolx, Browp[k], e1, col_lo); more && e < e1; e += 4) {
+            // the tile-row's first segment starts at B row k's first entry
+            const int eb = sg.y == 0 ? Browp[k] : lower_bound_i(Bcolx, Browp[k], e1, col_lo);
+            for (int e = eb; more && e < e1; e += 4) {
                 int cols[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) cols[v] = e + v < e1 ? Bcolx[e + v] : 0x7fffffff;
@@ -924,7 +926,9 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                 const int k = op.Acolx[x];
                 a = op.Avalx[x];
                 e1 = Browp[k + 1];
-                e0 = lower_bound_i(Bcolx, Browp[k], e1, col_lo);
+                // every entry of B row k is in a C tile of the tile-row, so
+                // none lies before the row's first segment
+                e0 = sg.w == 0 ? Browp[k] : lower_bound_i(Bcolx, Browp[k], e1, col_lo);
             }
             int pc[16];
             double pv[16];
