@@ -1106,7 +1106,7 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
               const int2 *__restrict__ items, const int *__restrict__ nitems, const int *__restrict__ Cptr,
               const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
               const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, double *__restrict__ Cval,
-              int *__restrict__ cursor, long long cur_stride) {
+              int *__restrict__ cursor, long long cur_stride, int *__restrict__ next_item) {
     __shared__ double s_acc[HEAVY_PART_COLS];
     __shared__ double s_sv[64 * HEAVY_STAGE];  // staged B values, source-major
     __shared__ int s_sc[64 * HEAVY_STAGE];     // their columns
@@ -1115,7 +1115,13 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
     const int lane = threadIdx.x;
     int *cur = cursor + (long long)blockIdx.x * cur_stride;
     const int ni = *nitems;
-    for (int h = blockIdx.x; h < ni; h += gridDim.x) {
+    // items are taken from a shared counter, so waves that drew short items
+    // take more (every wave exits once the counter passes ni)
+    for (;;) {
+        int h = 0;
+        if (lane == 0) h = atomicAdd(next_item, 1);
+        h = __shfl(h, 0, 64);
+        if (h >= ni) break;
         const int2 it = items[h];
         const int i = it.x, p0 = (int)((unsigned)it.y & 0xffffu), p1 = (int)((unsigned)it.y >> 16);
         const int I = i >> 4, r = i & 15;
@@ -1691,6 +1697,7 @@ struct SpgemmWork {
     int nheavy, nitems, hwaves;
     int *cursor;           // hwaves cursor arrays of hstride ints
     long long hstride;
+    int *hnext;            // work-item counter (zeroed before the launch)
 };
 
 static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -1739,6 +1746,7 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.nheavy = w.nitems = w.hwaves = 0;
     w.cursor = nullptr;
     w.hstride = 0;
+    w.hnext = nullptr;
     return w;
 }
 
@@ -1822,7 +1830,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
             hipLaunchKernelGGL(k_step3_heavy, dim3(w.hwaves), dim3(64), 0, s, op.Arowp, op.Acolx, op.Avalx,
                                (const int *)B->rowx_ptr, (const int *)B->rowx_col, (const double *)B->rowx_val,
                                (const int2 *)w.hitems, (const int *)w.hcount, C->tile_ptr, C->tile_colidx, C->mask,
-                               C->csr_ptr, C->tile_nnz, C->val, w.cursor, w.hstride);
+                               C->csr_ptr, C->tile_nnz, C->val, w.cursor, w.hstride, w.hnext);
             TSG_LAUNCH_CHECK();
         }
     } else if (P.all || P.s3_unfit) {
@@ -1947,8 +1955,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                 const int m = C->m;
                 TSG_TRY(tsg_alloc_t(ctx, &hflag, (size_t)m + 16));
                 TSG_TRY(tsg_alloc_t(ctx, &hcnt, (size_t)m + 1));
-                TSG_TRY(tsg_alloc_t(ctx, &hinfo, 2));
-                TSG_HIP(hipMemsetAsync(hinfo, 0, 8, s));
+                TSG_TRY(tsg_alloc_t(ctx, &hinfo, 4));  // {longest heavy A row, heavy rows, item counter, pad}
+                TSG_HIP(hipMemsetAsync(hinfo, 0, 16, s));
                 hipLaunchKernelGGL(k_heavy_flags, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, op.Arowp, op.Acolx,
                                    (const int *)B->rowx_ptr, (const unsigned char *)w.rowcls, C->tile_ptr,
                                    C->tile_colidx, m,
@@ -1983,6 +1991,7 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                     w.nheavy = (int)nrows;
                     w.nitems = (int)ni;
                     w.cursor = hcur;
+                    w.hnext = hinfo + 2;
                 }
             }
         }
