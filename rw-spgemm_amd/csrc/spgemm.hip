@@ -93,25 +93,40 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
                 pre[lane] = incl - len;
                 pb0[lane] = b0;
                 tsg_wave_sync();
+                // 4 B tile columns per lane are loaded before their bits are set
                 if (tot >= 64 * 32) {
                     // long B tile-rows (32+ tiles on average): the 64 lanes sweep
                     // one B tile-row at a time, no candidate search
                     const int na = min(64, a1 - c0);
                     for (int j = 0; j < na; j++) {
                         const int bj = pb0[j], lj = (j + 1 < na ? pre[j + 1] : tot) - pre[j];
-                        for (int c = lane; c < lj; c += 64) {
-                            const int d = Bcol[bj + c] - w0;
-                            if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                        for (int c = lane; c < lj; c += 256) {
+                            int d[4];
+#pragma unroll
+                            for (int v = 0; v < 4; v++) d[v] = c + 64 * v < lj ? Bcol[bj + c + 64 * v] - w0 : -1;
+#pragma unroll
+                            for (int v = 0; v < 4; v++)
+                                if (d[v] >= 0 && d[v] < 32 * BMW) atomicOr(&bm[d[v] >> 5], 1u << (d[v] & 31));
                         }
                     }
                 } else {
-                    for (int c = lane; c < tot; c += 64) {
-                        int lo = 0;
-    #pragma unroll
-                        for (int st = 32; st > 0; st >>= 1)
-                            if (pre[lo + st] <= c) lo += st;
-                        const int d = Bcol[pb0[lo] + (c - pre[lo])] - w0;
-                        if (d >= 0 && d < 32 * BMW) atomicOr(&bm[d >> 5], 1u << (d & 31));
+                    for (int c = lane; c < tot; c += 256) {
+                        int d[4];
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int cc = c + 64 * v;
+                            d[v] = -1;
+                            if (cc < tot) {
+                                int lo = 0;
+#pragma unroll
+                                for (int st = 32; st > 0; st >>= 1)
+                                    if (pre[lo + st] <= cc) lo += st;
+                                d[v] = Bcol[pb0[lo] + (cc - pre[lo])] - w0;
+                            }
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            if (d[v] >= 0 && d[v] < 32 * BMW) atomicOr(&bm[d[v] >> 5], 1u << (d[v] & 31));
                     }
                 }
                 tsg_wave_sync();
