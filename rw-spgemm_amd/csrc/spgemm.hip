@@ -1177,24 +1177,29 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                     s_n[lane] = cnt | (more ? 256 : 0);
                 }
                 int next = cb + cnt;  // lane q: where source q's next part starts
+                // only sources with entries in the part take a step
+                unsigned long long act = __ballot(lane < n && cnt > 0);
                 tsg_wave_sync();
-                // the staged data of source q+1 is read before source q's
-                // accumulation, which then costs one LDS round trip
-                int sn = s_n[0], sc = 0;
-                double a = s_a[0], sv = 0.0;
+                // the staged data of the next active source is read before this
+                // source's accumulation, which then costs one LDS round trip
+                int q = act ? __ffsll((long long)act) - 1 : 0;
+                int sn = act ? s_n[q] : 0, sc = 0;
+                double a = act ? s_a[q] : 0.0, sv = 0.0;
                 if (lane < (sn & 255)) {
-                    sc = s_sc[lane];
-                    sv = s_sv[lane];
+                    sc = s_sc[q * HEAVY_STAGE + lane];
+                    sv = s_sv[q * HEAVY_STAGE + lane];
                 }
-                for (int q = 0; q < n; q++) {  // ascending k; entries of one k have distinct columns
+                while (act) {  // ascending k; entries of one k have distinct columns
+                    act &= act - 1;
+                    const int q1 = act ? __ffsll((long long)act) - 1 : 0;
                     int sn1 = 0, sc1 = 0;
                     double a1 = 0.0, sv1 = 0.0;
-                    if (q + 1 < n) {
-                        sn1 = s_n[q + 1];
-                        a1 = s_a[q + 1];
+                    if (act) {
+                        sn1 = s_n[q1];
+                        a1 = s_a[q1];
                         if (lane < (sn1 & 255)) {
-                            sc1 = s_sc[(q + 1) * HEAVY_STAGE + lane];
-                            sv1 = s_sv[(q + 1) * HEAVY_STAGE + lane];
+                            sc1 = s_sc[q1 * HEAVY_STAGE + lane];
+                            sv1 = s_sv[q1 * HEAVY_STAGE + lane];
                         }
                     }
                     if (lane < (sn & 255)) s_acc[sc - lo] = __builtin_fma(a, sv, s_acc[sc - lo]);
@@ -1223,6 +1228,7 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                         if (lane == q) next = e;
                     }
                     tsg_wave_sync();  // the next k reads what this one wrote
+                    q = q1;
                     sn = sn1;
                     sc = sc1;
                     a = a1;
