@@ -88,6 +88,18 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     # element-wise numeric step with B's row index: for every tile-row, and
     # for the tile-rows the row-wise tile kernel cannot hold
     ctx.build_rowindex(tb)
+    # first with an A that has no row index (A's entries decoded from its tiles)
+    da2, ta2 = device_tiles(ctx, a)
+    for mode in (2, 0):
+        t3 = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
+        t3.element_mode = mode
+        tc3, _ = ctx.spgemm(ta2, tb, t3)
+        C3 = ctx.download_tiles(tc3)
+        for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
+            assert np.array_equal(C3[k], C[k]), f"element mode {mode}, A tiles only, differs in {k}"
+        ctx.free(tc3)
+    ctx.free(ta2)
+    ctx.free(da2)
     # with A's row index too, and rows above 64 products computed one wave
     # each (the heavy-row path: column parts of 4096 columns in LDS, one run
     # each; with items of ~256 products, runs of column parts go to
