@@ -343,13 +343,33 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 #define G2_THREADS 256
 #define G2_UMAX 512    // C tiles of one segment held in LDS
 #define G2_PCHUNK 256  // A tiles per enumeration round
+#define G2_MAP_MAX 16384  // widest tile-row span with a direct column map (bytes of LDS)
+
+// Most C tiles of a tile-row (capped at the step-2 segment size) and the
+// widest tile-column span of a tile-row: out[0], out[1] (atomic maxima).
+__global__ void __launch_bounds__(256)
+k_tilerow_stats(const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem, int *__restrict__ out) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= tilem) return;
+    const int c0 = Cptr[I], c1 = Cptr[I + 1];
+    if (c1 > c0) {
+        atomicMax(&out[0], min(c1 - c0, G2_UMAX));
+        atomicMax(&out[1], Ccol[c1 - 1] - Ccol[c0] + 1);
+    }
+}
 
 __global__ void __launch_bounds__(G2_THREADS)
 k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restrict__ nseg, const int *__restrict__ Cptr,
              const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
-             int *__restrict__ Cnnz) {
-    __shared__ int s_ccol[G2_UMAX];
-    __shared__ unsigned s_m[G2_UMAX * 8];  // two 16-bit rows per word, even row high
+             int *__restrict__ Cnnz, int umax, int mapbytes) {
+    // dynamic LDS, sized by the largest segment: the segment's C tile masks
+    // (two 16-bit rows per word, even row high), its tile columns, and when
+    // every segment has at most 256 tiles spanning at most mapbytes tile
+    // columns, a direct map tile column -> slot
+    extern __shared__ __align__(16) unsigned char g2_smem[];
+    unsigned *s_m = reinterpret_cast<unsigned *>(g2_smem);
+    int *s_ccol = reinterpret_cast<int *>(g2_smem + 32 * (size_t)umax);
+    unsigned char *s_map = g2_smem + 36 * (size_t)umax;
     __shared__ int s_pre[G2_PCHUNK];
     __shared__ int s_b0[G2_PCHUNK];
     __shared__ int s_tot;
@@ -365,9 +385,13 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     const bool whole = sg.y == 0 && sg.z == Cptr[I + 1] - Cptr[I];
-    for (int i = tid; i < u; i += G2_THREADS) s_ccol[i] = Ccol[c0 + i];
-    for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
     const int jlo = Ccol[c0], jhi = Ccol[c0 + u - 1];
+    for (int i = tid; i < u; i += G2_THREADS) {
+        const int J = Ccol[c0 + i];
+        s_ccol[i] = J;
+        if (mapbytes) s_map[J - jlo] = (unsigned char)i;
+    }
+    for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
     unsigned short *bst = s_bst[tid];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
@@ -401,13 +425,18 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             for (int st = G2_PCHUNK / 2; st > 0; st >>= 1)
                 if (p + st < G2_PCHUNK && s_pre[p + st] <= x) p += st;
             const int q = s_b0[p] + (x - s_pre[p]);
-            const int J = op.Bcol[q];
-            int lo = 0, hi = u - 1;  // J is in the segment's step-1 list
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+            const int J = op.Bcol[q];  // J is in the segment's step-1 list
+            int slot;
+            if (mapbytes) {
+                slot = s_map[J - jlo];
+            } else {
+                int lo = 0, hi = u - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
+                }
+                slot = lo;
             }
-            const int slot = lo;
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q * 16);
             const uint4 bA = bm4[0], bB = bm4[1];
             *reinterpret_cast<uint4 *>(bst) = bA;
@@ -1803,9 +1832,11 @@ static int launch_step2(hipStream_t s, const TileOperands &op, const tsg_dtiles 
         TSG_LAUNCH_CHECK();
     }
     TSG_TRY(launch_segments<0>(s, C, w, -1, G2_UMAX, 0, w.seg2, P.s3_elem ? w.rowcls : nullptr));
-    hipLaunchKernelGGL(k_step2_rows, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), 0, s, op, (const int4 *)w.seg2,
+    const int s2u = P.s2u > 0 ? P.s2u : G2_UMAX;
+    const size_t lds2 = 36 * (size_t)s2u + (((size_t)P.s2map + 15) & ~(size_t)15);
+    hipLaunchKernelGGL(k_step2_rows, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), lds2, s, op, (const int4 *)w.seg2,
                        (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
-                       C->tile_nnz);
+                       C->tile_nnz, s2u, P.s2map);
     TSG_LAUNCH_CHECK();
     if (P.s3_elem) {
         hipLaunchKernelGGL(k_step2_elem, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), 0, s, op, (const int *)B->rowx_ptr,
@@ -1939,6 +1970,20 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, keep));
     tsg_free(ctx, keep);
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
+    // step-2 LDS sizing: the largest segment and the widest tile-row span
+    P.s2u = G2_UMAX;
+    P.s2map = 0;
+    if (numtileC > 0) {
+        int *st2 = reinterpret_cast<int *>(static_cast<char *>(scr) + 32);  // zeroed with scr
+        hipLaunchKernelGGL(k_tilerow_stats, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, C->tile_ptr,
+                           C->tile_colidx, C->tilem, st2);
+        TSG_LAUNCH_CHECK();
+        long long v = 0;
+        TSG_TRY(tsg_read_scalar_i64(ctx, st2, 1, &v));
+        const int umax = (int)(v & 0xffffffffLL), span = (int)(v >> 32);
+        P.s2u = std::max(1, umax);
+        P.s2map = (umax <= 256 && span <= G2_MAP_MAX) ? span : 0;
+    }
     // ---- step 2
     const TileOperands op = make_operands(A, B);
     TSG_TRY(launch_step2(s, op, B, C, P, w));
