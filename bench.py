@@ -229,8 +229,10 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
     U[4,32]), 20% on R-MAT (scale 18, edge factor 16), all registered once
     (resident in HBM) under a 64 GiB budget.  Reports latency percentiles
     (arrival -> result complete in HBM) at a fixed rate, the burst capacity
-    (all requests at once) and, on the same request list, one plain call
-    after another on the same resident operands."""
+    (all requests at once) and, on the same request list, one call after
+    another on the same resident operands (prepared like the server's, with
+    the element row index) and plain calls on operands with the drop-in
+    preparation (tiles + CSC)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
@@ -280,6 +282,40 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
         for o in t:
             ctx.free(o)
     ctx.trim()
+    # sequential plain calls: operands with the drop-in preparation only
+    # (tiles + CSC, no element row index), one timed call per distinct
+    # (matrix, op) of the request list, summed over the requests
+    plain_ms = {}
+    for i in sorted(set(pick)):
+        d = ctx.upload(mats[i])
+        ta = ctx.csr2tile(d)
+        ctx.build_csc(ta)
+        tb = None
+        for op in sorted(set(o for j, o in zip(pick, ops) if j == i)):
+            if op == ts.OP_A2:
+                b = ta
+            else:
+                if tb is None:
+                    dt = ctx.transpose(d)
+                    tb = ctx.csr2tile(dt)
+                    ctx.build_csc(tb)
+                    ctx.free(dt)
+                b = tb
+            bounds = ctx.chunk_plan(ta, b, 1500000000)[0]
+            ctx.sync()
+            t0 = time.perf_counter()
+            for r0, r1 in zip(bounds, bounds[1:]):
+                c, _ = ctx.spgemm_tile_rows(ta, b, r0, r1, timed=False)
+                ctx.free(c)
+            ctx.sync()
+            plain_ms[(i, op)] = (time.perf_counter() - t0) * 1e3
+        for o in (tb, ta, d):
+            if o is not None:
+                ctx.free(o)
+        ctx.trim()
+    plain_s = sum(plain_ms[(i, op)] for i, op in zip(pick, ops)) / 1e3
+    progress(f"serve: sequential plain calls {plain_s:.1f} s for {n_req} requests "
+             f"({len(plain_ms)} distinct products timed)")
     ctx.close()
     srv = ts.Server(0, ts.server_config(budget_bytes=budget_gib << 30))
     t0 = time.perf_counter()
@@ -313,6 +349,10 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
             "burst_solves_per_s": round(cap, 2), "burst_p99_ms": round(float(np.percentile(blat, 99)), 2),
             "sequential_solves_per_s": round(n_req / seq_s, 2),
             "speedup_vs_sequential": round(seq_s / burst_s, 2),
+            "sequential_plain_solves_per_s": round(n_req / plain_s, 3),
+            "speedup_vs_sequential_plain": round(plain_s / burst_s, 2),
+            "sequential_plain_method": "one timed plain call (tiles + CSC, chunked) per distinct (matrix, op), "
+                                       "summed over the request list",
             "fused_launches": st1.fused_launches - st0.fused_launches,
             "single_launches": st1.single_launches - st0.single_launches,
             "peak_reserved_gib": round(st1.peak_reserved_bytes / 2 ** 30, 2),
