@@ -197,6 +197,13 @@ static void upload_tiles(tsg_ctx *c, const SMatrix *s, int col_major, tsg_dtiles
         CK("alloc", tsg_dmalloc(c, (void **)&t->csc_tileid, nt * 4 + 16));
         CK("iota", tsg_d_iota(c, t->csc_tileid, (long long)nt));
     }
+    /* The matrix's own CSR (rows ascending, checked when its tiles were
+     * built) is the element row index the hypersparse kernels walk. */
+    if (s->rowpointer && s->columnindex && s->value) {
+        UP(rowx_ptr, s->rowpointer, ((size_t)s->m + 1) * 4);
+        UP(rowx_col, s->columnindex, (size_t)s->nnz * 4);
+        UP(rowx_val, s->value, (size_t)s->nnz * 8);
+    }
 #undef UP
 }
 
@@ -216,6 +223,12 @@ void tilespgemm(SMatrix *matrixA, SMatrix *matrixB, SMatrix *matrixC, unsigned i
     upload_tiles(c, matrixA, 0, &A);
     upload_tiles(c, matrixB, 1, &Bcsc);
     CK("B to row-major tile order", tsg_tiles_from_csc_order(c, &Bcsc, &B));
+    /* B's element row index does not depend on the tile order */
+    B.rowx_ptr = Bcsc.rowx_ptr;
+    B.rowx_col = Bcsc.rowx_col;
+    B.rowx_val = Bcsc.rowx_val;
+    Bcsc.rowx_ptr = Bcsc.rowx_col = NULL;
+    Bcsc.rowx_val = NULL;
     tsg_tiles_free(c, &Bcsc);
     double tmin = 1e300;
     tsg_stats st, best;
