@@ -1342,26 +1342,31 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 #define G3_LOAD(j, S)                                                                                         \
     do {                                                                                                      \
         const int lb_ = SROW(4, j), q0_ = SROW(3, j);                                                         \
+        /* Every lane loads from an in-bounds address (clamped; the lanes past  \
+           the stage's data load a valid element and the store skips them), so \
+           the loads are straight-line code and the store of this stage, two   \
+           K-steps later, waits for these loads only. */                       \
         if (wave == 0) {                                                                                      \
-            if (lane < 2 * lb_) S##wa = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q0_ * 16)[lane]; \
-            if (lane < lb_) S##wb = reinterpret_cast<const uint4 *>(op.Bptr16 + (long long)q0_ * 16)[lane];   \
-            if (lane <= lb_) S##iv = op.Bnnz[q0_ + lane] - SROW(5, j);                                        \
+            const long long bq_ = lb_ > 0 ? (long long)q0_ * 16 : 0;                                          \
+            S##wa = reinterpret_cast<const uint4 *>(op.Bmask + bq_)[min(lane, max(2 * lb_ - 1, 0))];          \
+            S##wb = reinterpret_cast<const uint4 *>(op.Bptr16 + bq_)[min(lane, max(lb_ - 1, 0))];             \
+            S##iv = op.Bnnz[q0_ + min(lane, lb_)]; /* made relative at the store */                           \
         } else if (wave == 1) {                                                                               \
-            const int pan_ = SROW(2, j), pa0_ = SROW(1, j);                                                   \
-            if (lane < pan_) S##d0 = op.Aval[pa0_ + lane];                                                    \
-            if (lane + 64 < pan_) S##d1 = op.Aval[pa0_ + lane + 64];                                          \
-            if (lane + 128 < pan_) S##d2 = op.Aval[pa0_ + lane + 128];                                        \
-            if (lane + 192 < pan_) S##d3 = op.Aval[pa0_ + lane + 192];                                        \
-            if (lane < 16) {                                                                                  \
-                const long long x_ = (long long)SROW(0, j) * 16 + lane;                                       \
-                S##iv = (int)op.Amask[x_] | ((int)op.Aptr16[x_] << 16);                                       \
-            }                                                                                                 \
+            const int pan_ = SROW(2, j), pa0_ = SROW(1, j); /* A tiles are never empty */                    \
+            S##d0 = op.Aval[pa0_ + min(lane, pan_ - 1)];                                                      \
+            S##d1 = op.Aval[pa0_ + min(lane + 64, pan_ - 1)];                                                 \
+            S##d2 = op.Aval[pa0_ + min(lane + 128, pan_ - 1)];                                                \
+            S##d3 = op.Aval[pa0_ + min(lane + 192, pan_ - 1)];                                                \
+            const long long x_ = (long long)SROW(0, j) * 16 + (lane & 15);                                    \
+            S##wa.x = op.Amask[x_]; /* raw: combined at the store */                                          \
+            S##wa.y = op.Aptr16[x_];                                                                          \
         } else {                                                                                              \
-            const int pbn_ = SROW(6, j), pb0_ = SROW(5, j), b_ = (wave - 2) * 64 + lane;                      \
-            if (b_ < pbn_) S##d0 = op.Bval[pb0_ + b_];                                                        \
-            if (b_ + 128 < pbn_) S##d1 = op.Bval[pb0_ + b_ + 128];                                            \
-            if (b_ + 256 < pbn_) S##d2 = op.Bval[pb0_ + b_ + 256];                                            \
-            if (b_ + 384 < pbn_) S##d3 = op.Bval[pb0_ + b_ + 384];                                            \
+            const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
+            const int pb0_ = pbn_ > 0 ? SROW(5, j) : 0, pbl_ = max(pbn_ - 1, 0);                              \
+            S##d0 = op.Bval[pb0_ + min(b_, pbl_)];                                                            \
+            S##d1 = op.Bval[pb0_ + min(b_ + 128, pbl_)];                                                      \
+            S##d2 = op.Bval[pb0_ + min(b_ + 256, pbl_)];                                                      \
+            S##d3 = op.Bval[pb0_ + min(b_ + 384, pbl_)];                                                      \
         }                                                                                                     \
     } while (0)
 
@@ -1371,7 +1376,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
         if (wave == 0) {                                                                                      \
             if (lane < 2 * lb_) reinterpret_cast<uint4 *>(s_bm + (buf) * qb * 16)[lane] = S##wa;               \
             if (lane < lb_) reinterpret_cast<uint4 *>(s_bp + (buf) * qb * 16)[lane] = S##wb;                   \
-            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = S##iv;                                         \
+            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = S##iv - SROW(5, j);                            \
             /* nonempty-row bits of this half tile, OR-ed with the partner half */                            \
             const unsigned wd_[4] = {S##wa.x, S##wa.y, S##wa.z, S##wa.w};                                     \
             unsigned nz_ = 0;                                                                                 \
@@ -1389,10 +1394,10 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             if (lane + 64 < pan_) av_[lane + 64] = S##d1;                                                     \
             if (lane + 128 < pan_) av_[lane + 128] = S##d2;                                                   \
             if (lane + 192 < pan_) av_[lane + 192] = S##d3;                                                   \
-            const unsigned am_ = lane < 16 ? (unsigned)S##iv & 0xffffu : 0u;                                  \
+            const unsigned am_ = lane < 16 ? S##wa.x & 0xffffu : 0u;                                          \
             if (lane < 16) {                                                                                  \
                 s_am[buf][lane] = (unsigned short)am_;                                                        \
-                s_ap[buf][lane] = (unsigned char)((unsigned)S##iv >> 16);                                     \
+                s_ap[buf][lane] = (unsigned char)S##wa.y;                                                     \
             }                                                                                                 \
             const unsigned rows_ = (unsigned)__ballot(am_ != 0) & 0xffffu;                                    \
             if (am_) s_rows[buf][__popc(rows_ & ((1u << lane) - 1))] = (unsigned char)lane;                   \
