@@ -1309,17 +1309,19 @@ struct G3Layout {
 __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
     const size_t sz[G3_NARR] = {
         8u * (size_t)L.nnzmax,            // 0 acc
-        2u * 8u * (size_t)L.bnnz,         // 1 B values (2 stages)
+        // 1 B values (2 stages); before the first stage, the C tile columns
+        // (int) and their slot ids (u8) of all the row's C tiles
+        2u * 8u * (size_t)L.bnnz > 5u * G3_UMAX ? 2u * 8u * (size_t)L.bnnz : 5u * G3_UMAX,
         2u * 8u * (size_t)L.amax,         // 2 A values
-        4u * (size_t)L.umax,              // 3 C tile columns
-        4u * (size_t)L.umax,              // 4 C tile offsets
+        0u,                               // 3 (C tile columns share 1)
+        2u * (size_t)L.umax,              // 4 C tile offsets (u16)
         32u * (size_t)L.umax,             // 5 C masks
         16u * (size_t)L.umax,             // 6 C row offsets
-        32u * (size_t)L.lamax,            // 7 row table (8 ints per A tile)
+        28u * (size_t)L.lamax,            // 7 row table (7 ints per A tile)
         (size_t)L.lamax * (size_t)L.qb,   // 8 pair slots
         2u * 32u * (size_t)L.qb,          // 9 B masks
         2u * 16u * (size_t)L.qb,          // 10 B row offsets
-        2u * 4u * ((size_t)L.qb + 1),     // 11 B tile nnz offsets
+        2u * 2u * ((size_t)L.qb + 1),     // 11 B tile nnz offsets (u16)
         2u * 2u * (size_t)L.qb};          // 12 B nonempty-row bits
     size_t o = 0;
     for (int i = 0; i < G3_NARR; i++) {
@@ -1333,11 +1335,12 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 // wave-uniform: wave 0 loads the B tile masks (uint4 = 8 rows, 2 per tile),
 // row offsets (uint4 per tile) and nnz offsets; wave 1 the A tile (values,
 // masks, row offsets); waves 2-3 the B values (4 per lane).  A stage is
-// loaded three K-steps before it is computed (register ring of 3).
+// loaded three K-steps before it is computed (register ring of 3).  The
+// roles share one set of 10 registers per stage: wave 0 keeps its two
+// 16-byte words as raw bits in d0:d1 and d2:d3 (moved, never computed on).
 #define G3_SLOT_DECL(S)                                                   \
-    uint4 S##wa = make_uint4(0u, 0u, 0u, 0u), S##wb = make_uint4(0u, 0u, 0u, 0u); \
     double S##d0 = 0.0, S##d1 = 0.0, S##d2 = 0.0, S##d3 = 0.0;            \
-    int S##iv = 0
+    unsigned S##e0 = 0u, S##e1 = 0u
 
 #define G3_LOAD(j, S)                                                                                         \
     do {                                                                                                      \
@@ -1348,18 +1351,19 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
            K-steps later, waits for these loads only. */                       \
         if (wave == 0) {                                                                                      \
             const long long bq_ = lb_ > 0 ? (long long)q0_ * 16 : 0;                                          \
-            S##wa = reinterpret_cast<const uint4 *>(op.Bmask + bq_)[min(lane, max(2 * lb_ - 1, 0))];          \
-            S##wb = reinterpret_cast<const uint4 *>(op.Bptr16 + bq_)[min(lane, max(lb_ - 1, 0))];             \
-            S##iv = op.Bnnz[q0_ + min(lane, lb_)]; /* made relative at the store */                           \
+            const double2 m_ = reinterpret_cast<const double2 *>(op.Bmask + bq_)[min(lane, max(2 * lb_ - 1, 0))]; \
+            const double2 o_ = reinterpret_cast<const double2 *>(op.Bptr16 + bq_)[min(lane, max(lb_ - 1, 0))]; \
+            S##d0 = m_.x; S##d1 = m_.y; S##d2 = o_.x; S##d3 = o_.y;                                           \
+            S##e0 = op.Bnnz[q0_ + min(lane, lb_)]; /* made relative at the store */                           \
         } else if (wave == 1) {                                                                               \
             const int pan_ = SROW(2, j), pa0_ = SROW(1, j); /* A tiles are never empty */                    \
             S##d0 = op.Aval[pa0_ + min(lane, pan_ - 1)];                                                      \
             S##d1 = op.Aval[pa0_ + min(lane + 64, pan_ - 1)];                                                 \
             S##d2 = op.Aval[pa0_ + min(lane + 128, pan_ - 1)];                                                \
             S##d3 = op.Aval[pa0_ + min(lane + 192, pan_ - 1)];                                                \
-            const long long x_ = (long long)SROW(0, j) * 16 + (lane & 15);                                    \
-            S##wa.x = op.Amask[x_]; /* raw: combined at the store */                                          \
-            S##wa.y = op.Aptr16[x_];                                                                          \
+            const long long x_ = (long long)(a0 + (j)) * 16 + (lane & 15); /* A tile a0 + j */              \
+            S##e0 = op.Amask[x_];                                                                             \
+            S##e1 = op.Aptr16[x_];                                                                            \
         } else {                                                                                              \
             const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
             const int pb0_ = pbn_ > 0 ? SROW(5, j) : 0, pbl_ = max(pbn_ - 1, 0);                              \
@@ -1374,11 +1378,12 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
     do {                                                                                                      \
         const int lb_ = SROW(4, j);                                                                           \
         if (wave == 0) {                                                                                      \
-            if (lane < 2 * lb_) reinterpret_cast<uint4 *>(s_bm + (buf) * qb * 16)[lane] = S##wa;               \
-            if (lane < lb_) reinterpret_cast<uint4 *>(s_bp + (buf) * qb * 16)[lane] = S##wb;                   \
-            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = S##iv - SROW(5, j);                            \
+            if (lane < 2 * lb_) reinterpret_cast<double2 *>(s_bm + (buf) * qb * 16)[lane] = make_double2(S##d0, S##d1); \
+            if (lane < lb_) reinterpret_cast<double2 *>(s_bp + (buf) * qb * 16)[lane] = make_double2(S##d2, S##d3); \
+            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = (unsigned short)((int)S##e0 - SROW(5, j));     \
             /* nonempty-row bits of this half tile, OR-ed with the partner half */                            \
-            const unsigned wd_[4] = {S##wa.x, S##wa.y, S##wa.z, S##wa.w};                                     \
+            const unsigned wd_[4] = {(unsigned)__double2loint(S##d0), (unsigned)__double2hiint(S##d0),        \
+                                     (unsigned)__double2loint(S##d1), (unsigned)__double2hiint(S##d1)};       \
             unsigned nz_ = 0;                                                                                 \
             _Pragma("unroll") for (int i_ = 0; i_ < 4; i_++) {                                                \
                 const int k_ = (lane & 1) * 8 + 2 * i_;                                                       \
@@ -1394,10 +1399,10 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             if (lane + 64 < pan_) av_[lane + 64] = S##d1;                                                     \
             if (lane + 128 < pan_) av_[lane + 128] = S##d2;                                                   \
             if (lane + 192 < pan_) av_[lane + 192] = S##d3;                                                   \
-            const unsigned am_ = lane < 16 ? S##wa.x & 0xffffu : 0u;                                          \
+            const unsigned am_ = lane < 16 ? S##e0 & 0xffffu : 0u;                                            \
             if (lane < 16) {                                                                                  \
                 s_am[buf][lane] = (unsigned short)am_;                                                        \
-                s_ap[buf][lane] = (unsigned char)S##wa.y;                                                     \
+                s_ap[buf][lane] = (unsigned char)S##e1;                                                       \
             }                                                                                                 \
             const unsigned rows_ = (unsigned)__ballot(am_ != 0) & 0xffffu;                                    \
             if (am_) s_rows[buf][__popc(rows_ & ((1u << lane) - 1))] = (unsigned char)lane;                   \
@@ -1412,7 +1417,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
         }                                                                                                     \
     } while (0)
 
-#define SROW(k, j) s_row[(k) * lamax + (j)]
+#define SROW(k, j) s_row[((k) - 1) * lamax + (j)]  // k = 1..7
 
 __global__ void __launch_bounds__(G3_THREADS)
 k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
@@ -1426,22 +1431,24 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     double *s_acc = reinterpret_cast<double *>(g3_smem + off[0]);
     double *s_bval = reinterpret_cast<double *>(g3_smem + off[1]);
     double *s_aval = reinterpret_cast<double *>(g3_smem + off[2]);
-    int *s_ccol = reinterpret_cast<int *>(g3_smem + off[3]);
-    int *s_coff = reinterpret_cast<int *>(g3_smem + off[4]);
+    int *s_ccol = reinterpret_cast<int *>(g3_smem + off[1]);  // setup only: the B value area
+    unsigned char *s_cidx = g3_smem + off[1] + 4 * G3_UMAX;    // setup only
+    unsigned short *s_coff = reinterpret_cast<unsigned short *>(g3_smem + off[4]);  // < G3_NNZMAX
     unsigned short *s_cmask = reinterpret_cast<unsigned short *>(g3_smem + off[5]);
     unsigned char *s_cptr = g3_smem + off[6];
     int *s_row = reinterpret_cast<int *>(g3_smem + off[7]);
     unsigned char *s_pslot = g3_smem + off[8];
     unsigned short *s_bm = reinterpret_cast<unsigned short *>(g3_smem + off[9]);
     unsigned char *s_bp = g3_smem + off[10];
-    int *s_bnnz = reinterpret_cast<int *>(g3_smem + off[11]);
+    unsigned short *s_bnnz = reinterpret_cast<unsigned short *>(g3_smem + off[11]);
     unsigned short *s_bnz = reinterpret_cast<unsigned short *>(g3_smem + off[12]);
     __shared__ unsigned short s_am[2][16];
     __shared__ unsigned char s_ap[2][16];
     __shared__ unsigned char s_rows[2][16];
     __shared__ int s_nr[2];
     __shared__ int s_npair;
-    __shared__ unsigned short s_queue[G3_THREADS / 64][128];  // per-wave compacted task list
+    __shared__ int s_wne[G3_THREADS / 64];
+    static_assert(G3_UMAX <= G3_THREADS, "one C tile per thread in the setup");
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int I = (int)tsg_xcd_block(blockIdx.x, gridDim.x);
@@ -1449,14 +1456,19 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
     const int nbase = Cnnz[c0], nrow = Cnnz[c0 + u] - nbase;
     const int a0 = op.Aptr[I], la = op.Aptr[I + 1] - a0;
-    for (int i = tid; i < u; i += G3_THREADS) {
-        s_ccol[i] = Ccol[c0 + i];
-        s_coff[i] = Cnnz[c0 + i] - nbase;
-        const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + i) * 16);
-        reinterpret_cast<uint4 *>(s_cmask)[2 * i] = cm[0];
-        reinterpret_cast<uint4 *>(s_cmask)[2 * i + 1] = cm[1];
-        reinterpret_cast<uint4 *>(s_cptr)[i] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + i) * 16)[0];
+    // C tiles of the tile-row (u <= G3_UMAX), one per thread.  Only the
+    // nonempty ones get accumulator metadata (offset, masks, row offsets),
+    // at their rank among the nonempty tiles: a task is active only when
+    // one of its products exists, so no task lands in an empty C tile.
+    int ccol = 0, cnz0 = 0;
+    bool ne = false;
+    if (tid < u) {
+        ccol = Ccol[c0 + tid];
+        cnz0 = Cnnz[c0 + tid];
+        ne = Cnnz[c0 + tid + 1] > cnz0;
     }
+    const unsigned long long neb = __ballot(ne);
+    if (lane == 0) s_wne[wave] = __popcll(neb);
     // row table: one entry per A tile of the tile-row (wave 0, la <= 64)
     if (wave == 0) {
         int lb = 0;
@@ -1465,7 +1477,6 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
             const int pa0 = op.Annz[p], q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
             const int pb0 = op.Bnnz[q0];
             lb = q1 - q0;
-            SROW(0, lane) = p;
             SROW(1, lane) = pa0;
             SROW(2, lane) = op.Annz[p + 1] - pa0;
             SROW(3, lane) = q0;
@@ -1478,18 +1489,33 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (lane == 63) s_npair = incl;
     }
     __syncthreads();
+    int ci = __popcll(neb & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; w++) ci += s_wne[w];
+    int un = 0;
+#pragma unroll
+    for (int w = 0; w < G3_THREADS / 64; w++) un += s_wne[w];
     // C tile slot of every (A tile, B tile) pair of the tile-row.  When the
     // row's C tile columns span at most the accumulator's bytes, the
     // accumulator area first holds a direct map column -> slot (every pair's
     // column is in the row's C tile list, so unmapped bytes are never read);
-    // otherwise a binary search over the sorted C tile columns.
-    const int jlo = s_ccol[0], jspan = s_ccol[u - 1] - jlo + 1;
+    // otherwise a binary search over the sorted C tile columns.  An empty
+    // tile maps to the next nonempty tile's slot (< 256); no task uses it.
+    const int jlo = Ccol[c0], jspan = Ccol[c0 + u - 1] - jlo + 1;
     const bool direct = jspan <= 8 * L.nnzmax;
     unsigned char *s_map = reinterpret_cast<unsigned char *>(s_acc);
-    if (direct) {
-        for (int i = tid; i < u; i += G3_THREADS) s_map[s_ccol[i] - jlo] = (unsigned char)i;
-        __syncthreads();
+    if (tid < u) {
+        s_ccol[tid] = ccol;
+        s_cidx[tid] = (unsigned char)ci;
+        if (direct) s_map[ccol - jlo] = (unsigned char)ci;
+        if (ne) {
+            s_coff[ci] = (unsigned short)(cnz0 - nbase);
+            const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + tid) * 16);
+            reinterpret_cast<uint4 *>(s_cmask)[2 * ci] = cm[0];
+            reinterpret_cast<uint4 *>(s_cmask)[2 * ci + 1] = cm[1];
+            reinterpret_cast<uint4 *>(s_cptr)[ci] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + tid) * 16)[0];
+        }
     }
+    __syncthreads();
     for (int x = tid; x < s_npair; x += G3_THREADS) {
         int j = 0;
 #pragma unroll
@@ -1504,7 +1530,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
                 const int mid = (lo + hi) >> 1;
                 if (s_ccol[mid] < J) lo = mid + 1; else hi = mid;
             }
-            s_pslot[x] = (unsigned char)lo;
+            s_pslot[x] = s_cidx[lo];
         }
     }
     __syncthreads();  // the map area becomes the accumulator
@@ -1529,8 +1555,14 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         const unsigned magic = nr > 1 ? (unsigned)((0x100000000ULL + nr - 1) / (unsigned)nr) : 0u;
         const unsigned short *sbz = s_bnz + buf * qb;
         // candidate tasks (B tile qi, A row rr) meeting at least one nonempty
-        // B row, compacted per wave (at most two chunks of 64 per wave)
-        int qn = 0;
+        // B row, compacted per wave into a register queue: position Q of the
+        // queue is lane Q & 63 of qlo (Q < 64) or qhi (Q >= 64).  A wave has
+        // at most two chunks of 64 candidates (ncand <= G3_QB * 16 = 512), so
+        // qn <= 64 before each chunk.  Each chunk is moved by one forward
+        // permute over all 64 lanes: active lanes to the queue positions
+        // qn.., inactive ones after them, so every lane receives exactly one
+        // value and the permute needs no LDS memory.
+        int qn = 0, qlo = 0, qhi = 0;
         for (int c = wave * 64; c < ncand; c += G3_THREADS) {
             const int x = c + lane;
             int code = 0;
@@ -1542,18 +1574,21 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
                 code = (qi << 4) | rr;
             }
             const unsigned long long bal = __ballot(act);
-            if (act) s_queue[wave][qn + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)code;
-            qn += __popcll(bal);
+            const int na = __popcll(bal), below = __popcll(bal & ((1ull << lane) - 1ull));
+            const int rank = act ? below : na + (lane - below);
+            const int got = __builtin_amdgcn_ds_permute(((qn + rank) & 63) << 2, code);
+            if (lane >= qn && lane < qn + na) qlo = got;
+            if (lane + 64 < qn + na) qhi = got;
+            qn += na;
         }
-        tsg_wave_sync();
         const unsigned short *sbm = s_bm + buf * qb * 16;
         const unsigned char *sbp = s_bp + buf * qb * 16;
-        const int *sbnnz = s_bnnz + buf * (qb + 1);
+        const unsigned short *sbnnz = s_bnnz + buf * (qb + 1);
         const double *sbv = s_bval + buf * bnnzs;
         const double *sav = s_aval + buf * amax;
         for (int t0 = 0; t0 < qn; t0 += 64) {
             if (t0 + lane >= qn) break;
-            const int code = s_queue[wave][t0 + lane];
+            const int code = t0 ? qhi : qlo;
             const int qi = code >> 4, rr = code & 15;
             const int slot = s_pslot[pbase + qi];
             const unsigned cm = s_cmask[slot * 16 + rr];
@@ -1596,7 +1631,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 #undef G3_ITER
     // write out: values are already in final order; columns from the masks
     for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
-    for (int x = tid; x < u * 16; x += G3_THREADS) {
+    for (int x = tid; x < un * 16; x += G3_THREADS) {
         const int slot = x >> 4;
         unsigned cm = s_cmask[x];
         unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
@@ -1613,7 +1648,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 // Step-3 class of every C tile-row: 1 = row-wise kernel (its C tiles, C
 // nonzeros, A tiles and B tile-rows are within the G3 caps), 0 = tile-parallel
 // kernel, 2 = nothing to compute.  out[0] += class-0 rows; out[1..6] = max
-// over class-1 rows of (C tiles, C nonzeros, A tiles, B tiles per B
+// over class-1 rows of (nonempty C tiles, C nonzeros, A tiles, B tiles per B
 // tile-row, B nonzeros per B tile-row, A tile nonzeros) for the LDS layout.
 __global__ void __launch_bounds__(256)
 k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
@@ -1641,7 +1676,9 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
             }
             k = ok ? 1 : 0;
             if (ok) {
-                mu = c1 - c0; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
+                int ne = 0;  // nonempty C tiles (the row-wise kernel's slots)
+                for (int t = c0; t < c1; t++) ne += Cnnz[t + 1] > Cnnz[t];
+                mu = ne; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
             }
         }
         cls[I] = k;
