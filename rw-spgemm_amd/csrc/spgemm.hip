@@ -857,8 +857,10 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // ascending K, bits in ascending column) and, for each a_ik, its lanes
 // scatter the entries of B row k that fall in the segment's column range:
 // distinct C entries per step, steps in ascending k, so each C entry again
-// sums its products in the serial SPA order, without atomics.
-#define G3E_NNZ 2048
+// sums its products in the serial SPA order, without atomics.  1536
+// nonzeros keep the block's LDS (accumulator, hash, column bytes) under
+// 40,960 B, four blocks per CU in gfx950's 1,280-byte LDS granules.
+#define G3E_NNZ 1536
 
 // LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
 // stored + 1 (0 = empty), value = the entry's position in the segment.
