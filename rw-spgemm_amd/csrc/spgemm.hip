@@ -1491,6 +1491,16 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (lane == 63) s_npair = incl;
     }
     __syncthreads();
+    // the ring's first three stages need only the row table: their loads
+    // are in flight while the C tile slots and pair slots are set up
+    G3_SLOT_DECL(R0_);
+    G3_SLOT_DECL(R1_);
+    G3_SLOT_DECL(R2_);
+    if (la > 0) {
+        G3_LOAD(0, R0_);
+        if (la > 1) G3_LOAD(1, R1_);
+        if (la > 2) G3_LOAD(2, R2_);
+    }
     int ci = __popcll(neb & ((1ull << lane) - 1ull));
     for (int w = 0; w < wave; w++) ci += s_wne[w];
     int un = 0;
@@ -1537,15 +1547,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     }
     __syncthreads();  // the map area becomes the accumulator
     for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
-    G3_SLOT_DECL(R0_);
-    G3_SLOT_DECL(R1_);
-    G3_SLOT_DECL(R2_);
-    if (la > 0) {
-        G3_LOAD(0, R0_);
-        if (la > 1) G3_LOAD(1, R1_);
-        if (la > 2) G3_LOAD(2, R2_);
-        G3_STORE(0, R0_, 0);
-    }
+    if (la > 0) G3_STORE(0, R0_, 0);
     __syncthreads();
 
     // compute K-step j from buffer j&1
