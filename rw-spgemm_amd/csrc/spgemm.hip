@@ -1654,12 +1654,22 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 // kernel, 2 = nothing to compute.  out[0] += class-0 rows; out[1..6] = max
 // over class-1 rows of (nonempty C tiles, C nonzeros, A tiles, B tiles per B
 // tile-row, B nonzeros per B tile-row, A tile nonzeros) for the LDS layout.
+// 16 lanes per tile-row (16 tile-rows per block): the lanes walk the row's A
+// tiles and C tiles in strides of 16, so the dependent loads of different A
+// tiles are in flight together; one set of atomics per block.
+__device__ __forceinline__ int g16_max(int v) {
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 16));
+    return v;
+}
 __global__ void __launch_bounds__(256)
 k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
             unsigned char *__restrict__ cls, int *__restrict__ out, int none_fit) {
-    const int I = blockIdx.x * 256 + threadIdx.x;
+    __shared__ int s_red[4][7];
+    const int r = threadIdx.x & 15, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int I = blockIdx.x * 16 + (threadIdx.x >> 4);
     int unfit = 0, mu = 0, mn = 0, ml = 0, mq = 0, mb = 0, ma = 0;
-    if (I < tilem) {
+    if (I < tilem) {  // uniform in the 16-lane group
         const int c0 = Cptr[I], c1 = Cptr[I + 1];
         const int nrow = Cnnz[c1] - Cnnz[c0];
         unsigned char k;
@@ -1668,33 +1678,49 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
         } else {
             const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
             bool ok = !none_fit && c1 - c0 <= G3_UMAX && nrow <= G3_NNZMAX && a1 - a0 <= G3_LAMAX;
-            int lq = 0, lbn = 0, lan = 0;
-            for (int p = a0; ok && p < a1; p++) {
-                const int K = op.Acol[p];
-                const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
-                const int bn = op.Bnnz[q1] - op.Bnnz[q0];
-                ok = q1 - q0 <= G3_QB && bn <= G3_BNNZ;
-                lq = max(lq, q1 - q0);
-                lbn = max(lbn, bn);
-                lan = max(lan, op.Annz[p + 1] - op.Annz[p]);
+            if (ok) {  // group-uniform
+                int bad = 0, lq = 0, lbn = 0, lan = 0, ne = 0;
+                for (int p = a0 + r; p < a1; p += 16) {
+                    const int K = op.Acol[p];
+                    const int q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
+                    const int bn = op.Bnnz[q1] - op.Bnnz[q0];
+                    bad |= q1 - q0 > G3_QB || bn > G3_BNNZ;
+                    lq = max(lq, q1 - q0);
+                    lbn = max(lbn, bn);
+                    lan = max(lan, op.Annz[p + 1] - op.Annz[p]);
+                }
+                // nonempty C tiles (the row-wise kernel's slots)
+                for (int t = c0 + r; t < c1; t += 16) ne += Cnnz[t + 1] > Cnnz[t];
+                ok = g16_max(bad) == 0;
+                ne = tsg_group16_sum(ne);
+                lq = g16_max(lq); lbn = g16_max(lbn); lan = g16_max(lan);
+                if (ok) {
+                    mu = ne; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
+                }
             }
             k = ok ? 1 : 0;
-            if (ok) {
-                int ne = 0;  // nonempty C tiles (the row-wise kernel's slots)
-                for (int t = c0; t < c1; t++) ne += Cnnz[t + 1] > Cnnz[t];
-                mu = ne; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
-            }
         }
-        cls[I] = k;
-        unfit = k == 0;
+        if (r == 0) cls[I] = k;
+        unfit = r == 0 && k == 0;
     }
     unfit = tsg_wave_sum(unfit);
     mu = tsg_wave_max(mu); mn = tsg_wave_max(mn); ml = tsg_wave_max(ml);
     mq = tsg_wave_max(mq); mb = tsg_wave_max(mb); ma = tsg_wave_max(ma);
-    if ((threadIdx.x & 63) == 0) {
-        if (unfit) atomicAdd(&out[0], unfit);
-        atomicMax(&out[1], mu); atomicMax(&out[2], mn); atomicMax(&out[3], ml);
-        atomicMax(&out[4], mq); atomicMax(&out[5], mb); atomicMax(&out[6], ma);
+    if (lane == 0) {
+        s_red[wave][0] = unfit; s_red[wave][1] = mu; s_red[wave][2] = mn; s_red[wave][3] = ml;
+        s_red[wave][4] = mq; s_red[wave][5] = mb; s_red[wave][6] = ma;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const int i = threadIdx.x;
+        int v = s_red[0][i];
+#pragma unroll
+        for (int w = 1; w < 4; w++) v = i == 0 ? v + s_red[w][i] : max(v, s_red[w][i]);
+        if (i == 0) {
+            if (v) atomicAdd(&out[0], v);
+        } else {
+            atomicMax(&out[i], v);
+        }
     }
 }
 
@@ -1895,7 +1921,7 @@ static int launch_step2(hipStream_t s, const TileOperands &op, const tsg_dtiles 
 
 static int launch_row_class(hipStream_t s, const TileOperands &op, tsg_dtiles *C, unsigned char *rowcls,
                             int *cls_out, int none_fit) {
-    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
+    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 16)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
                        C->tilem, rowcls, cls_out, none_fit);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
