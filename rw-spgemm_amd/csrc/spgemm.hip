@@ -867,8 +867,15 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 #define G3E_EHASH 4096  // >= 2 * G3E_NNZ
 #define G3E_PF 8         // C tiles per thread whose metadata is loaded per round
 __device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 2654435761u) >> 20; }  // 12 bits
+// PK: when every tile-row spans fewer than 8192 tile columns the key (+1) is
+// below 2^21 and one 32-bit slot holds key << 11 | position (positions are
+// below G3E_NNZ <= 2048); without the separate position array the block's
+// LDS fits five blocks per CU.
+#define G3E_PK_SPAN 8192
+static_assert(G3E_NNZ <= 2048, "positions take 11 bits of a packed hash slot");
 
-__global__ void __launch_bounds__(256)
+template <bool PK>
+__global__ void __launch_bounds__(256, PK ? 5 : 1)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
              const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
@@ -876,7 +883,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
              double *__restrict__ Cval, const unsigned char *__restrict__ heavy) {
     __shared__ double s_acc[G3E_NNZ];
     __shared__ unsigned s_ekey[G3E_EHASH];
-    __shared__ unsigned short s_epos[G3E_EHASH];
+    __shared__ unsigned short s_epos[PK ? 1 : G3E_EHASH];
     __shared__ unsigned char s_ecol[G3E_NNZ];  // in-tile column of each entry, final order
     const int tid = threadIdx.x;
     // the grid is an upper bound: the first *nseg blocks (spread evenly over
@@ -926,8 +933,13 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     cm &= ~(1u << bb);
                     const unsigned key = (((j16[v] + 15 - bb) << 4) | (unsigned)rr) + 1u;
                     unsigned h = ent_hash(key);
-                    while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
-                    s_epos[h] = (unsigned short)pos;
+                    if (PK) {
+                        const unsigned ent = key << 11 | (unsigned)pos;
+                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = (h + 1) & (G3E_EHASH - 1);
+                    } else {
+                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
+                        s_epos[h] = (unsigned short)pos;
+                    }
                     s_ecol[pos] = (unsigned char)(15 - bb);
                     pos++;
                 }
@@ -950,10 +962,19 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
         const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
         unsigned h = ent_hash(key);
         int probe = 0;
-        while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
-        if (probe < G3E_EHASH) {
-            double *acc = &s_acc[s_epos[h]];
-            *acc = __builtin_fma(a, b, *acc);
+        if (PK) {
+            unsigned ent;
+            while (((ent = s_ekey[h]) >> 11) != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
+            if (probe < G3E_EHASH) {
+                double *acc = &s_acc[ent & 2047u];
+                *acc = __builtin_fma(a, b, *acc);
+            }
+        } else {
+            while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
+            if (probe < G3E_EHASH) {
+                double *acc = &s_acc[s_epos[h]];
+                *acc = __builtin_fma(a, b, *acc);
+            }
         }
     };
     if (row < m && nseg_nnz > 0 && op.Arowp && !skip) {
@@ -1945,7 +1966,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
     }
     if (!P.all && P.s3_unfit && P.s3_elem) {
         TSG_TRY(launch_segments<1>(s, C, w, 0, 0x7fffffff, G3E_NNZ, w.seg3, w.rowcls));
-        hipLaunchKernelGGL(k_step3_elem, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
+        hipLaunchKernelGGL(P.s3pk ? k_step3_elem<true> : k_step3_elem<false>, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
                            C->tile_nnz, C->csr_col, C->val, (const unsigned char *)w.hflag);
@@ -2055,6 +2076,7 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         const int umax = (int)(v & 0xffffffffLL), span = (int)(v >> 32);
         P.s2u = std::max(1, umax);
         P.s2map = (umax <= 256 && span <= G2_MAP_MAX) ? span : 0;
+        P.s3pk = span < G3E_PK_SPAN ? 1 : 0;
     }
     // ---- step 2
     const TileOperands op = make_operands(A, B);

@@ -133,6 +133,7 @@ struct tsg_spgemm_plan {
     int s1w;        // step-1 bitmap window, 32-bit words per wave
     int s2u;        // most C tiles of a step-2 tile-pair segment (LDS sizing)
     int s2map;      // bytes of the step-2 column -> slot map (0: binary search)
+    int s3pk;       // element step 3 packs hash keys and positions (narrow tile-rows)
 };
 int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune,
                        tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out);

@@ -147,6 +147,22 @@ def test_wide_step1_window_rmat(ctx):
     spgemm_check(ctx, a, t(a), label="rmat14 AAT windows")
 
 
+def test_wide_b_element_hash_unpacked(ctx):
+    """C = A*B with B 3000 x 200000 (12,500 tile columns, random entries):
+    C's tile-rows span more than 8192 tile columns, so the element-wise
+    numeric kernel keeps its hash keys and positions in separate arrays
+    (narrower products pack both into one 32-bit slot); every array of C
+    is compared with the oracle through all element-wise modes."""
+    rng = np.random.default_rng(77)
+    a = ts.gen_er(3000, 6.0, seed=77)
+    m, n, nnz = 3000, 200000, 30000
+    key = np.unique(rng.integers(0, m, nnz).astype(np.int64) * n + rng.integers(0, n, nnz))
+    r, c = (key // n).astype(np.int32), (key % n).astype(np.int32)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=m))]).astype(np.int32)
+    b = ts.Csr(m, n, rp, c, rng.uniform(-1, 1, len(c)))
+    spgemm_check(ctx, a, b, label="ER3000 x wide B (200000 columns)")
+
+
 def test_bad_step1_window_rejected(ctx):
     a = ts.gen_stencil5(32, 32)
     da = ctx.upload(a)
