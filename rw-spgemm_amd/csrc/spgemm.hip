@@ -500,6 +500,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
 // columns).  Work is one step per intermediate product instead of one per
 // (A tile, B tile) pair; OR is order-free, so LDS atomics are exact.
 #define G2E_HASH 1024  // >= 2 * G2_UMAX
+static_assert(G2_UMAX <= 512, "C tile index in 9 bits of a step-2 hash slot");
 
 __device__ __forceinline__ unsigned col_hash10(int J) { return ((unsigned)J * 2654435761u) >> 22; }
 
@@ -509,8 +510,10 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
              const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
              int *__restrict__ Cnnz) {
     __shared__ unsigned s_m[G2_UMAX * 8];  // two 16-bit rows per word, even row high
-    __shared__ int s_hkey[G2E_HASH];
-    __shared__ unsigned short s_hval[G2E_HASH];
+    // hash of the segment's tile columns: slot (J - jlo + 1) << 9 | C tile
+    // index (J - jlo + 1 < 2^23 as B has at most 2^27 - 32 columns when this
+    // kernel runs, index < G2_UMAX = 512); 0 = empty
+    __shared__ unsigned s_hkey[G2E_HASH];
     const int tid = threadIdx.x;
     const int ns = *nseg;
     if ((int)blockIdx.x >= ns) return;  // block-uniform
@@ -520,13 +523,14 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int I = sg.x;
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
-    for (int i = tid; i < G2E_HASH; i += G2_THREADS) s_hkey[i] = -1;
+    for (int i = tid; i < G2E_HASH; i += G2_THREADS) s_hkey[i] = 0u;
     __syncthreads();
+    const int jlo = Ccol[c0];
     for (int i = tid; i < u; i += G2_THREADS) {
         const int J = Ccol[c0 + i];
         unsigned h = col_hash10(J);
-        while (atomicCAS(&s_hkey[h], -1, J) != -1) h = (h + 1) & (G2E_HASH - 1);  // u <= G2E_HASH / 2
-        s_hval[h] = (unsigned short)i;
+        const unsigned ent = (unsigned)(J - jlo + 1) << 9 | (unsigned)i;
+        while (atomicCAS(&s_hkey[h], 0u, ent) != 0u) h = (h + 1) & (G2E_HASH - 1);  // u <= G2E_HASH / 2
     }
     __syncthreads();
     const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
@@ -593,11 +597,13 @@ k_step2_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     if (!more) continue;
                     const int J = col >> 4;
                     unsigned h = col_hash10(J);  // J is in the segment's tile list
+                    const unsigned key = (unsigned)(J - jlo + 1);
                     int probe = 0;
-                    while (s_hkey[h] != J && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
+                    unsigned ent;
+                    while (((ent = s_hkey[h]) >> 9) != key && ++probe < G2E_HASH) h = (h + 1) & (G2E_HASH - 1);
                     if (probe == G2E_HASH) continue;  // not reached for a consistent C structure
                     const unsigned bit = 1u << (15 - (col & 15));
-                    atomicOr(&s_m[(int)s_hval[h] * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
+                    atomicOr(&s_m[(int)(ent & 511u) * 8 + (r >> 1)], bit << ((r & 1) ? 0 : 16));
                 }
             }
         }
@@ -2014,7 +2020,9 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     P.dense_min = tune.dense_acc_min_nnz;
     // element path: B carries its row index, and C's columns fit the
     // 28-bit entry keys of the element kernel's LDS hash
-    P.s3_elem = B->rowx_ptr != nullptr && tune.element_mode != 1 && B->n < (1 << 27);
+    // element-wise keys: (column offset) << 4 in 32 bits, and tile columns
+    // below 2^23 - 1 for the step-2 hash slots
+    P.s3_elem = B->rowx_ptr != nullptr && tune.element_mode != 1 && B->n <= (1 << 27) - 32;
     P.elem_all = B->rowx_ptr != nullptr && tune.element_mode == 2;
     hipStream_t s = ctx->stream;
 
