@@ -125,6 +125,25 @@ void tilespgemm(SMatrix *matrixA, SMatrix *matrixB, SMatrix *matrixC,
  * B row they select (src/main.cu:391-397).  Computed on the device. */
 unsigned long long tsg_nnzcub(const SMatrix *A, const SMatrix *B);
 
+/* Matrix Market coordinate file -> CSR with the reference loader's
+ * semantics: symmetric/hermitian headers expanded to both triangles,
+ * pattern -> 1.0, duplicates kept, each row in file order (so unsorted
+ * unless the file is stored column-major); arrays malloc'd.  Returns 0, or
+ * -1 cannot open, -2 bad banner, -4 bad size line, -5 entry out of range
+ * or file too short.  Host only (no device work).
+ * Replaces: int mmio_allinone(...)  src/mmio_highlevel.h:593-759 */
+int mmio_allinone(int *m, int *n, MAT_PTR_TYPE *nnz, int *isSymmetric,
+                  MAT_PTR_TYPE **csrRowPtr, int **csrColIdx,
+                  MAT_VAL_TYPE **csrVal, char *filename);
+
+/* The reference driver, `tsg_driver -d <dev> -aat {0|1} <A> [-b <B>]
+ * [-reorder 1] [-reorder_algo rcm|degree|random|original|tasa|<adapter>]`
+ * with the flow of src/main.cu:15-638 (load, value override i % 10, B = A,
+ * A^T or the -b file, optional reorder plan + apply, nnzCub, csr2tile,
+ * tilespgemm, tile2csr, restore, check).  Returns the process exit code.
+ * Replaces: int main(int argc, char **argv)  src/main.cu:15 */
+int tsg_driver_main(int argc, char **argv);
+
 #ifdef __cplusplus
 }
 #endif

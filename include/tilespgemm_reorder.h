@@ -83,16 +83,32 @@ int axis_permutation_is_identity(const AxisPermutation *perm);
 void reorder_info_init(ReorderInfo *info);
 void reorder_info_destroy(ReorderInfo *info);
 
+/* Reads a new->old order file (one zero-based id per line, whitespace
+ * separated) of n ids into perm (both maps).  Returns 0, -10 cannot open,
+ * -11 fewer than n ids, -12 id outside [0, n), -13 duplicate id, -14 more
+ * than n ids (messages as the reference prints them).
+ * Replaces: read_external_order_file  src/external_reorder.h:53-98 */
+int read_external_order_file(const char *path, int n, AxisPermutation *perm);
+
 /* Ordering plan for an AA / AAT / AB problem with the axis sharing of
  * build_external_reorder_plan (src/external_reorder.h:204-319): AA and AAT
  * use A's order on all three axes, AB uses A's order for rows and the
- * inner axis and B's for columns.  `algorithm` is "rcm", "degree",
- * "random" (seed from TSG_RANDOM_SEED, default 1) or "original"; the
- * orders are computed on the device from the matrices themselves, so the
- * .mtx paths of the reference signature are not read (they may be NULL).
- * Returns the reference's codes: -1 bad arguments, -2 missing B, -3/-8
- * inner dimensions differ, -4/-5/-6/-7 non-square operand, -20 unknown
- * algorithm (the reference's "missing dependency" exit).
+ * inner axis and B's order for columns.  `algorithm` "rcm", "degree"
+ * (ids by descending row length, ties by id), "random" (seed from
+ * TSG_RANDOM_SEED, default 1) and "original" are computed on the device
+ * from the matrices themselves (the paths are then not read and may be
+ * NULL); any other name runs the reference's adapter
+ * $TASA_REORDER_ALGORITHM_ROOT/<algorithm>/order_gen <path> <order file>
+ * (root default "reorder_algorithm", order files under
+ * $TASA_REORDER_ORDER_DIR, default /tmp/tasa_reorder_orders) on
+ * matrix_path_A, and for AB on matrix_path_B (or A's path), and reads its
+ * order file back.  Status codes are the reference's: -1 bad arguments,
+ * -2 missing B, -3/-8 inner dimensions differ, -4/-5/-6/-7 non-square
+ * operand, -9 unknown problem kind, -10..-14 bad order file (see
+ * read_external_order_file), -20 adapter missing or failed (also: a
+ * device order with no device), -30 device order failed.  Every argument
+ * check happens before any device work.  The tile estimates are exact
+ * device counts, or -1 when no device is present.
  * Replaces: build_external_reorder_plan  src/external_reorder.h:204 */
 int build_external_reorder_plan(ReorderProblemKind kind, const char *algorithm, const char *matrix_path_A,
                                 const char *matrix_path_B, const SMatrix *matrixA, const SMatrix *matrixB,
@@ -105,19 +121,23 @@ int build_original_reorder_plan(ReorderProblemKind kind, const SMatrix *matrixA,
 
 /* The reference's TASA/GTSP heuristic planner is not implemented here;
  * this entry point builds the RCM plan of build_external_reorder_plan for
- * the problem instead (mode "rcm").
+ * the problem instead (mode "rcm").  Codes as the reference: -1 NULL
+ * argument, -3 missing B for AAT/AB, -2 unknown kind.
  * Replaces: int build_reorder_plan(...)  src/reorder.h:2407 */
 int build_reorder_plan(const ReorderProblem *problem, ReorderInfo *info);
 
 /* output = P*input*Q^T (row r <- input row row_perm->old_id[r], column c ->
  * col_perm->new_id[c], rows sorted), computed on the device; output arrays
- * malloc'd, output->isSymmetric = 0.  -1 on shape mismatch.
+ * malloc'd, output->isSymmetric = 0.  -1 on shape mismatch, -2 on a device
+ * failure.
  * Replaces: apply_bipermutation_csr  src/reorder.h:2429 */
 int apply_bipermutation_csr(const SMatrix *input, const AxisPermutation *row_perm,
                             const AxisPermutation *col_perm, SMatrix *output);
 
 /* In-place inverse of apply for a result C: row r -> row_perm->old_id[r],
  * column c -> col_perm->old_id[c], rows sorted; NULL perms are a no-op.
+ * The matrix's CSR arrays are freed and replaced by new malloc'd ones.
+ * -1 on shape mismatch or device failure.
  * Replaces: restore_bipermutation_csr  src/reorder.h:2506 */
 int restore_bipermutation_csr(SMatrix *matrix, const AxisPermutation *row_perm,
                               const AxisPermutation *col_perm);

@@ -63,6 +63,12 @@ enum {
     TSG_VAL_MOD10 = 3        /* value[i] = i % 10, src/main.cu:191-192 */
 };
 int tsg_hcsr_set_values(tsg_hcsr *h, int mode, unsigned long long seed);
+/* In place: each row stably sorted by column, duplicate entries summed in
+ * stored order (nnz shrinks; arrays are not reallocated). */
+int tsg_hcsr_canonicalize(tsg_hcsr *h);
+/* Matrix Market file -> canonical CSR: mmio_allinone (tilespgemm.h), then
+ * tsg_hcsr_canonicalize.  *is_symmetric = the header's symmetric flag. */
+int tsg_read_mtx(const char *path, tsg_hcsr *out, int *is_symmetric);
 
 /* 2D 5-point Laplacian on nx*ny (row = x + nx*y), diag 4, off-diag -1. */
 int tsg_gen_stencil5(int nx, int ny, tsg_hcsr *out);

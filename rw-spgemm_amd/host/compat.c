@@ -29,19 +29,26 @@ int tsg_compat_set_device(int device) {
     return 0;
 }
 
-tsg_ctx *tsg_compat_ctx(void) {
+/* The process-wide context, or NULL when no device can be opened. */
+tsg_ctx *tsg_compat_ctx_try(void) {
     if (!g_ctx) {
         int dev = g_dev;
         if (dev < 0) {
             const char *e = getenv("TSG_DEVICE");
             dev = e ? atoi(e) : 0;
         }
-        if (tsg_ctx_create(dev, &g_ctx) != TSG_OK) {
-            fprintf(stderr, "tilespgemm: cannot create device context: %s\n", tsg_last_error());
-            abort();
-        }
+        if (tsg_ctx_create(dev, &g_ctx) != TSG_OK) g_ctx = NULL;
     }
     return g_ctx;
+}
+
+tsg_ctx *tsg_compat_ctx(void) {
+    tsg_ctx *c = tsg_compat_ctx_try();
+    if (!c) {
+        fprintf(stderr, "tilespgemm: cannot create device context: %s\n", tsg_last_error());
+        abort();
+    }
+    return c;
 }
 
 /* Entry points returning void in the reference abort on failure. */

@@ -226,6 +226,49 @@ _sig("tsg_server_replay", _i, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_
 _sig("tsg_gen_batch_spec", _i, [_i, _ull, _i, _i, C.POINTER(BatchSpec)])
 _sig("tsg_gen_batch_problem", _i, [C.POINTER(BatchSpec), C.POINTER(HCsr)])
 _sig("tsg_compat_set_device", _i, [_i])
+_sig("tsg_hcsr_canonicalize", _i, [C.POINTER(HCsr)])
+_sig("mmio_allinone", _i, [C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(C.POINTER(_i)),
+                           C.POINTER(C.POINTER(_i)), C.POINTER(C.POINTER(_d)), C.c_char_p])
+_sig("tsg_driver_main", _i, [_i, C.POINTER(C.c_char_p)])
+
+
+class AxisPermutation(C.Structure):
+    """Same layout as src/reorder.h:29-34."""
+    _fields_ = [("n", _i), ("old_id", C.POINTER(_i)), ("new_id", C.POINTER(_i))]
+
+
+class ReorderInfo(C.Structure):
+    """Same layout as src/reorder.h:36-63."""
+    _fields_ = [("enabled", _i), ("n", _i), ("row_perm", AxisPermutation), ("inner_perm", AxisPermutation),
+                ("col_perm", AxisPermutation), ("input_tiles_A_before", _ll), ("input_tiles_A_after", _ll),
+                ("input_tiles_B_before", _ll), ("input_tiles_B_after", _ll), ("estimated_c_tiles_before", _ll),
+                ("estimated_c_tiles_after", _ll), ("reorder_time_ms", _d), ("signature_time_ms", _d),
+                ("tile_profile_time_ms", _d), ("affinity_time_ms", _d), ("sort_time_ms", _d),
+                ("local_refine_time_ms", _d), ("proxy_time_ms", _d), ("permute_time_ms", _d),
+                ("risk_input_tile_ratio", _d), ("risk_c_tile_ratio", _d), ("local_refine_window", _i),
+                ("proxy_sample_rows", _i), ("guard_degraded_to_identity", _i), ("structural_conservative", _i),
+                ("shared_symmetric_csr", _i), ("mode", C.c_char * 64), ("risk_reason", C.c_char * 128)]
+
+
+class ReorderProblem(C.Structure):
+    _fields_ = [("kind", _i), ("matrixA", C.POINTER(SMatrix)), ("matrixB", C.POINTER(SMatrix))]
+
+
+PROBLEM_AA, PROBLEM_AAT, PROBLEM_AB = 0, 1, 2
+
+_sig("axis_permutation_init", None, [C.POINTER(AxisPermutation)])
+_sig("axis_permutation_destroy", None, [C.POINTER(AxisPermutation)])
+_sig("axis_permutation_alloc", _i, [C.POINTER(AxisPermutation), _i])
+_sig("reorder_info_init", None, [C.POINTER(ReorderInfo)])
+_sig("reorder_info_destroy", None, [C.POINTER(ReorderInfo)])
+_sig("read_external_order_file", _i, [C.c_char_p, _i, C.POINTER(AxisPermutation)])
+_sig("build_external_reorder_plan", _i, [_i, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(SMatrix),
+                                         C.POINTER(SMatrix), C.POINTER(ReorderInfo), C.POINTER(_d)])
+_sig("build_original_reorder_plan", _i, [_i, C.POINTER(SMatrix), C.POINTER(SMatrix), C.POINTER(ReorderInfo)])
+_sig("build_reorder_plan", _i, [C.POINTER(ReorderProblem), C.POINTER(ReorderInfo)])
+_sig("apply_bipermutation_csr", _i, [C.POINTER(SMatrix), C.POINTER(AxisPermutation), C.POINTER(AxisPermutation),
+                                     C.POINTER(SMatrix)])
+_sig("restore_bipermutation_csr", _i, [C.POINTER(SMatrix), C.POINTER(AxisPermutation), C.POINTER(AxisPermutation)])
 
 
 def _check(status):
@@ -464,10 +507,124 @@ def set_values(csr, mode, seed=1):
 
 
 def read_mtx(path):
+    """Canonical CSR of a Matrix Market file (rows sorted, duplicates
+    summed) and the header's symmetric flag."""
     h = HCsr()
     sym = _i(0)
     _check(lib.tsg_read_mtx(path.encode(), C.byref(h), C.byref(sym)))
     return _take_hcsr(h), bool(sym.value)
+
+
+def mmio_allinone(path):
+    """The reference loader (src/mmio_highlevel.h:593-759): returns
+    (status, csr, symmetric); rows in file order, duplicates kept."""
+    m, n, nnz, sym = _i(0), _i(0), _i(0), _i(0)
+    rp, ci, va = C.POINTER(_i)(), C.POINTER(_i)(), C.POINTER(_d)()
+    st = lib.mmio_allinone(C.byref(m), C.byref(n), C.byref(nnz), C.byref(sym), C.byref(rp), C.byref(ci),
+                           C.byref(va), path.encode())
+    if st != 0:
+        return st, None, False
+    h = HCsr(m.value, n.value, nnz.value, rp, ci, va)
+    return 0, _take_hcsr(h), bool(sym.value)
+
+
+def canonicalize(csr: Csr) -> Csr:
+    """Rows sorted by column, duplicates summed (tsg_hcsr_canonicalize)."""
+    rp, ci, va = csr.rowptr.copy(), csr.colidx.copy(), csr.val.copy()
+    h = HCsr(csr.m, csr.n, csr.nnz, rp.ctypes.data_as(C.POINTER(_i)), ci.ctypes.data_as(C.POINTER(_i)),
+             va.ctypes.data_as(C.POINTER(_d)))
+    _check(lib.tsg_hcsr_canonicalize(C.byref(h)))
+    return Csr(csr.m, csr.n, rp, ci[:h.nnz], va[:h.nnz])
+
+
+class HostSMatrix:
+    """An SMatrix whose CSR fields point at numpy arrays kept alive here."""
+
+    def __init__(self, csr: Csr, symmetric=False):
+        self.csr = csr
+        self.s = SMatrix()
+        self.s.m, self.s.n, self.s.nnz, self.s.isSymmetric = csr.m, csr.n, csr.nnz, int(symmetric)
+        self.s.rowpointer = csr.rowptr.ctypes.data_as(C.POINTER(_i))
+        self.s.columnindex = csr.colidx.ctypes.data_as(C.POINTER(_i))
+        self.s.value = csr.val.ctypes.data_as(C.POINTER(_d))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def perm_arrays(p: AxisPermutation):
+    """(old_id, new_id) numpy copies of an AxisPermutation."""
+    if p.n <= 0 or not p.old_id:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32)
+    return (np.ctypeslib.as_array(p.old_id, shape=(p.n,)).copy(),
+            np.ctypeslib.as_array(p.new_id, shape=(p.n,)).copy())
+
+
+def read_order_file(path, n):
+    """read_external_order_file: (status, old_id, new_id)."""
+    p = AxisPermutation()
+    lib.axis_permutation_init(C.byref(p))
+    try:
+        st = lib.read_external_order_file(path.encode() if path is not None else None, n, C.byref(p))
+        old, new = perm_arrays(p) if st == 0 else (None, None)
+        return st, old, new
+    finally:
+        lib.axis_permutation_destroy(C.byref(p))
+
+
+def reorder_plan(kind, algorithm, a: HostSMatrix | None, b: HostSMatrix | None = None, path_a=None, path_b=None,
+                 builder="external"):
+    """build_external_reorder_plan / build_original_reorder_plan /
+    build_reorder_plan: (status, dict of the ReorderInfo fields with numpy
+    permutations, ordergen_ms)."""
+    info = ReorderInfo()
+    lib.reorder_info_init(C.byref(info))
+    ms = _d(0.0)
+    pa = a.ref if a is not None else None
+    pb = b.ref if b is not None else None
+    try:
+        if builder == "external":
+            st = lib.build_external_reorder_plan(kind, algorithm.encode() if algorithm is not None else None,
+                                                 path_a.encode() if path_a else None,
+                                                 path_b.encode() if path_b else None, pa, pb, C.byref(info),
+                                                 C.byref(ms))
+        elif builder == "original":
+            st = lib.build_original_reorder_plan(kind, pa, pb, C.byref(info))
+        else:
+            prob = ReorderProblem(kind, C.pointer(a.s) if a is not None else None,
+                                  C.pointer(b.s) if b is not None else None)
+            st = lib.build_reorder_plan(C.byref(prob), C.byref(info))
+        out = {k: getattr(info, k) for k, _ in ReorderInfo._fields_ if not k.endswith("_perm")}
+        out["mode"] = info.mode.decode()
+        out["risk_reason"] = info.risk_reason.decode()
+        for ax in ("row_perm", "inner_perm", "col_perm"):
+            out[ax] = perm_arrays(getattr(info, ax))
+        return st, out, ms.value
+    finally:
+        lib.reorder_info_destroy(C.byref(info))
+
+
+def driver(args):
+    """tsg_driver_main(["tsg_driver", *args]) in this process; returns its
+    exit code."""
+    argv = [b"tsg_driver"] + [str(x).encode() for x in args]
+    arr = (C.c_char_p * (len(argv) + 1))(*argv, None)
+    return lib.tsg_driver_main(len(argv), arr)
+
+
+def read_csr_bin(path) -> Csr:
+    """C written by the driver's -o option."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"TSGCSR1\0", raw[:8]
+    m, n, nnz = np.frombuffer(raw, np.int64, 3, 8)
+    o = 32
+    rp = np.frombuffer(raw, np.int32, m + 1, o)
+    o += 4 * (m + 1)
+    ci = np.frombuffer(raw, np.int32, nnz, o)
+    o += 4 * nnz
+    va = np.frombuffer(raw, np.float64, nnz, o)
+    return Csr(m, n, rp, ci, va)
 
 
 # ---------------------------------------------------------------- device

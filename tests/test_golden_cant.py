@@ -21,16 +21,23 @@ def test_fixture_is_consistent():
     assert abs(GOLD["nnzCub"] / GOLD["nnzC"] - GOLD["compression_rate"]) < 1e-5
 
 
+def test_read_mtx_returns_matrix_and_flag():
+    """The reader the cant.mtx checks use, on a committed fixture."""
+    a, sym = ts.read_mtx(os.path.join(HERE, "golden", "stencil5_8x8_sym.mtx"))
+    assert sym and (a.m, a.n, a.nnz) == (64, 64, 288)
+
+
 @needs_file
 def test_cant_read_counts():
-    a = ts.read_mtx(MTX)
+    a, _ = ts.read_mtx(MTX)
     assert (a.m, a.n, a.nnz) == (GOLD["m"], GOLD["n"], GOLD["nnzA"])
 
 
 @needs_file
 @pytest.mark.gpu
 def test_cant_a2_counts(ctx):
-    a = ts.set_values(ts.read_mtx(MTX), ts.VAL_MOD10)
+    a, _ = ts.read_mtx(MTX)
+    a = ts.set_values(a, ts.VAL_MOD10)
     da = ctx.upload(a)
     assert ctx.nnzcub(da, da) == GOLD["nnzCub"]
     ta = ctx.csr2tile(da)

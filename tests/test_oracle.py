@@ -12,6 +12,7 @@ Each check is chosen so that a plausible slip in the restatement fails:
   * transposes / permutations against SciPy, degree order against its
     one-line definition, RCM against invariants.
 """
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -303,3 +304,69 @@ def test_permute_roundtrip_and_scipy():
     assert np.array_equal(op, ref.indptr) and np.array_equal(oc, ref.indices) and np.array_equal(ov, ref.data)
     back = oracle.permute_csr(ts.Csr(80, 80, op, oc, ov), new, old)
     assert np.array_equal(back[0], a.rowptr) and np.array_equal(back[1], a.colidx) and np.array_equal(back[2], a.val)
+
+
+# ------------------------------------------------------------------ symmetrize / symmetric-header degree
+@pytest.mark.parametrize("n,density,seed", [(300, 0.02, 1), (257, 0.1, 2), (400, 0.004, 3)])
+def test_symmetrize_vs_scipy_on_nonsymmetric(n, density, seed):
+    """pattern(A + A^T) with sorted rows against SciPy's sparse sum of the
+    two patterns; the inputs are not symmetric, so a restatement that drops
+    the transpose half (or the A half) fails."""
+    a = sprand(n, n, density, seed)
+    s = _pattern(a)
+    assert (s != s.T).nnz > 0
+    ref = (s + s.T).tocsr()
+    ref.sort_indices()
+    gp, gc = oracle.symmetrize(a)
+    assert np.array_equal(gp, ref.indptr) and np.array_equal(gc, ref.indices)
+
+
+def _file_degrees(text):
+    """Degrees as reorder_algorithm/common/order_utils.py:30-45 counts them
+    from the file's entry lines: +1 for the row of every stored entry, +1
+    for its column too when the header is symmetric and the entry is off the
+    diagonal."""
+    lines = text.splitlines()
+    symmetric = "symmetric" in lines[0].lower()
+    body = [ln for ln in lines[1:] if ln.strip() and not ln.startswith("%")]
+    n = int(body[0].split()[0])
+    deg = [0] * n
+    for ln in body[1:]:
+        r, c = (int(x) - 1 for x in ln.split()[:2])
+        deg[r] += 1
+        if symmetric and r != c:
+            deg[c] += 1
+    return n, deg
+
+
+def _sym_files(tmp_path):
+    yield open(os.path.join(os.path.dirname(__file__), "golden", "stencil5_8x8_sym.mtx")).read()
+    rng = np.random.default_rng(9)
+    n = 120
+    ent = [(i, i) for i in range(0, n, 3)]
+    for _ in range(400):
+        r, c = sorted(rng.integers(0, n, size=2).tolist(), reverse=True)
+        ent.append((r, c))
+    ent.append(ent[5])  # a duplicate line counts twice in the file's degrees
+    body = "".join(f"{r + 1} {c + 1} 1.0\n" for r, c in ent)
+    yield f"%%MatrixMarket matrix coordinate real symmetric\n{n} {n} {len(ent)}\n{body}"
+
+
+def test_degree_order_symmetric_header_vs_file_definition(tmp_path):
+    """ora_degree_order on the stored triangle with symmetric_header=1, and
+    on the expanded CSR with 0, both equal sorted(range(n), key=(-deg, id))
+    with deg counted from the file's lines."""
+    for k, text in enumerate(_sym_files(tmp_path)):
+        n, deg = _file_degrees(text)
+        ref = sorted(range(n), key=lambda i: (-deg[i], i))
+        p = tmp_path / f"s{k}.mtx"
+        p.write_text(text)
+        st, expanded, sym = ts.mmio_allinone(str(p))
+        assert st == 0 and sym
+        q = tmp_path / f"t{k}.mtx"
+        q.write_text(text.replace("symmetric", "general", 1))
+        st, tri, sym = ts.mmio_allinone(str(q))
+        assert st == 0 and not sym
+        assert list(oracle.degree_order(tri, True)) == ref
+        assert list(oracle.degree_order(expanded, False)) == ref
+        assert list(oracle.degree_order(tri, False)) != ref  # the mirror counts
