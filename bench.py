@@ -38,6 +38,17 @@ def step3_algorithmic_bytes(ta, tc_numtile, tc_nnz, tilen):
     return a_bytes + b_bytes + c_meta + c_write
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(a, nnzcub_per_row, target_s=15.0):
     """The reference CPU path (src/spgemm_serialref_spa_new.h, count + fill
     passes, timed as src/external/cusparse/main.cu:191-222) restated in
@@ -69,7 +80,7 @@ def cpu_baseline(a, nnzcub_per_row, target_s=15.0):
         oracle.spgemm_rows(a, a, r0, r1, values=True)
         vt.append(time.perf_counter() - t0)
     return {"value": round(2.0 * cub / tmin / 1e9, 4), "unit": "GFLOPS", "cores": oracle.num_threads(),
-            "kind": "port",
+            "cpu_model": cpu_model(), "kind": "port",
             "sample": f"rows [{r0},{r1}) of the same A^2 ({r1 - r0} of {a.m} rows, {cub} products), "
                       f"count+fill passes, min of {len(times)} runs ({tmin:.2f} s)",
             "values_spa": {"value": round(2.0 * cub / min(vt) / 1e9, 4), "unit": "GFLOPS",
@@ -327,9 +338,21 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
     reg_s = time.perf_counter() - t0
     progress(f"serve: registered {len(mats)} matrices in {reg_s:.1f} s")
     hs = [handles[i] for i in pick]
+
+    def replay_progress(label):
+        """Prints the replay's own submissions / completions, at most every 10 s."""
+        last = [time.perf_counter()]
+
+        def cb(sub, done):
+            now = time.perf_counter()
+            if now - last[0] >= 10.0 or done == n_req:
+                progress(f"serve: {label}: {sub}/{n_req} submitted, {done} completed")
+                last[0] = now
+        return cb
+
     # burst: every request at t = 0 (capacity)
     t0 = time.perf_counter()
-    _, binfo = srv.replay(hs, ops, [0.0] * n_req, keep=False)
+    _, binfo = srv.replay(hs, ops, [0.0] * n_req, keep=False, progress=replay_progress("burst"))
     burst_s = time.perf_counter() - t0
     progress(f"serve: burst {burst_s:.2f} s")
     cap = n_req / burst_s
@@ -337,7 +360,7 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
     arr = np.cumsum(rng.exponential(1000.0 / rate, n_req)).tolist()
     st0 = srv.stats()
     t0 = time.perf_counter()
-    _, infos = srv.replay(hs, ops, arr, keep=False)
+    _, infos = srv.replay(hs, ops, arr, keep=False, progress=replay_progress(f"Poisson {rate:.2f} req/s"))
     open_s = time.perf_counter() - t0
     st1 = srv.stats()
     srv.close()
@@ -363,7 +386,37 @@ def serve_leg(ts, n_req=120, lam=None, seed=3, n_er=24, n_rmat=2, rmat_scale=18,
             "resident_gib": round(st1.resident_bytes / 2 ** 30, 2)}
 
 
-def rmat_leg(ts, scale=20, chunk_products=1500000000):
+def rmat_cpu_baseline(a, b, target_s=10.0):
+    """The oracle's faithful SPA (count + fill passes, per-row (nB/32+1)-word
+    bitmap, 64-bit row offsets since C overflows int32) on a contiguous
+    block of rows of the same A*A^T, grown until it takes about target_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    rp = a.rowptr.astype(np.int64)
+    rowlen_b = np.diff(b.rowptr.astype(np.int64))
+    cs = np.concatenate([[0], np.cumsum(rowlen_b[a.colidx])])
+    cub_row = cs[rp[1:]] - cs[rp[:-1]]
+    mid, rows = a.m // 2, 1024
+    while True:
+        r0, r1 = mid - rows // 2, mid + rows // 2
+        t, _ = oracle.spa_structure_rows_timed(a, b, r0, r1)
+        if t >= target_s / 4 or rows >= a.m // 2:
+            break
+        rows = min(a.m // 2, rows * 4)
+    times = [t]
+    for _ in range(2):
+        times.append(oracle.spa_structure_rows_timed(a, b, r0, r1)[0])
+    cub = int(cub_row[r0:r1].sum())
+    tmin = min(times)
+    return {"value": round(2.0 * cub / tmin / 1e9, 4), "unit": "GFLOPS", "cores": oracle.num_threads(),
+            "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"rows [{r0},{r1}) of A*A^T in the generator's order ({r1 - r0} of {a.m} rows, {cub} "
+                      f"products), count+fill passes with 64-bit row offsets, min of {len(times)} runs "
+                      f"({tmin:.2f} s)"}
+
+
+def rmat_leg(ts, scale=20, chunk_products=1500000000, cpu=True):
     """R-MAT scale-20 C = A*A^T (edge factor 16, (0.57, 0.19, 0.19, 0.05),
     symmetrized, duplicates merged, values U(0,1], seed 1) under the random,
     RCM and degree orderings, each computed on the device.  C (~2.5e10
@@ -420,6 +473,12 @@ def rmat_leg(ts, scale=20, chunk_products=1500000000):
     out["nnzC_invariant_across_orders"] = len(nnzc) == 1
     ctx.free(d)
     ctx.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        tp, tc, tv = oracle.transpose(a)
+        out["cpu_baseline"] = rmat_cpu_baseline(a, ts.Csr(a.n, a.m, tp, tc, tv))
+        progress(f"rmat{scale}: cpu baseline {out['cpu_baseline']['value']} GFLOPS")
     return out
 
 
@@ -440,6 +499,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-27d", action="store_true")
     ap.add_argument("--rmat-only", action="store_true")
+    ap.add_argument("--no-rmat", action="store_true")
     ap.add_argument("--rmat-scale", type=int, default=20)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -456,7 +516,7 @@ def main():
     import tilespgemm as ts
 
     if args.rmat_only:
-        print(json.dumps({"rmat": rmat_leg(ts, scale=args.rmat_scale)}), flush=True)
+        print(json.dumps({"rmat": rmat_leg(ts, scale=args.rmat_scale, cpu=not args.no_cpu)}), flush=True)
         return
     if args.serve_only:
         print(json.dumps({"served": serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)}),
@@ -551,6 +611,8 @@ def main():
     ctx.close()
     if not args.no_serve:
         out["served"] = serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)
+    if not args.no_rmat:
+        out["rmat"] = rmat_leg(ts, scale=args.rmat_scale, cpu=not args.no_cpu)
     if not args.no_cpu:
         rp = a.rowptr.astype(np.int64)
         rowlen = np.diff(rp)

@@ -153,6 +153,26 @@ typedef struct {
 int tsg_tiles_download(tsg_ctx *ctx, const tsg_dtiles *d, tsg_htiles *out);
 void tsg_htiles_free(tsg_htiles *h);
 
+/* Checksums of a C tile matrix, or of a tile-row view of one, so products
+ * too large to download can be compared: `entries` sums a 64-bit mix of
+ * (row, column, value bits) over C's nonzeros as tile2csr emits them,
+ * `tiles` a mix of (tile-row, tile-column, tile nonzeros, row masks, row
+ * offsets) over its tiles.  The sums are order-free, so the checksums of
+ * the tile-row chunks of a product add up (mod 2^64) to those of the
+ * whole.  Rows are numbered from row_base (a multiple of 16: the view's
+ * first row in the whole product), tile columns relative to jbase.
+ * `entries_t` is `entries` with each entry's row and column swapped: the
+ * two are equal when C is bit-symmetric (A*A^T).                       */
+typedef struct {
+    long long numtile, nnz;
+    unsigned long long tiles, entries, entries_t;
+} tsg_csum;
+int tsg_tiles_checksum(tsg_ctx *ctx, const tsg_dtiles *c, long long row_base, int jbase, tsg_csum *out);
+/* Rows rows[0..nrows) of a C tile matrix or view (ids local to it, any
+ * order, repeats allowed) as a host CSR of nrows rows (malloc'd; free with
+ * tsg_hcsr_free); columns relative to tile column jbase. */
+int tsg_tiles_rows(tsg_ctx *ctx, const tsg_dtiles *c, int jbase, const int *rows, int nrows, tsg_hcsr *out);
+
 /* ------------------------------------------------------------------ */
 /* SpGEMM C = A*B on device tiles.  B must carry its CSC tile index
  * (tsg_tiles_build_csc).  Passing the same object as A and B computes A^2
@@ -299,6 +319,9 @@ int tsg_batch_get_info(const tsg_batch *b, tsg_batch_info *out);
 /* Problem i's C (after a run): tile arrays in the problem's own indices
  * and/or CSR; either pointer may be NULL.  Synchronizes the stream. */
 int tsg_batch_result(tsg_batch *b, int i, tsg_htiles *tiles, tsg_hcsr *csr);
+/* Checksums (tsg_tiles_checksum) of problem i's C in its own row and
+ * column numbering, computed on the device.  Synchronizes the stream. */
+int tsg_batch_result_checksum(tsg_batch *b, int i, tsg_csum *out);
 void tsg_batch_destroy(tsg_batch *b);
 
 /* The batched workload: `count` problems from `seed`; n log-uniform in
@@ -369,6 +392,11 @@ int tsg_server_submit(tsg_server *s, int handle, int op, unsigned long long *id)
 /* Blocks until request id is done (timeout_ms <= 0: no limit); returns its status. */
 int tsg_server_wait(tsg_server *s, unsigned long long id, double timeout_ms, tsg_request_info *info);
 int tsg_server_result_csr(tsg_server *s, unsigned long long id, tsg_hcsr *out);
+/* Checksums (tsg_tiles_checksum) of a finished request's C, summed over
+ * its chunks or taken on its share of a fused product. */
+int tsg_server_result_checksum(tsg_server *s, unsigned long long id, tsg_csum *out);
+/* Rows rows[0..nrows) of a finished request's C as a host CSR. */
+int tsg_server_result_rows(tsg_server *s, unsigned long long id, const int *rows, int nrows, tsg_hcsr *out);
 int tsg_server_release(tsg_server *s, unsigned long long id);
 int tsg_server_stats(tsg_server *s, tsg_server_stats_t *out);
 /* Open-loop load: request i submitted at arrival_ms[i] after the call;
@@ -376,6 +404,14 @@ int tsg_server_stats(tsg_server *s, tsg_server_stats_t *out);
  * result on completion. */
 int tsg_server_replay(tsg_server *s, int n, const int *handles, const int *ops, const double *arrival_ms,
                       int keep, unsigned long long *ids, tsg_request_info *infos);
+/* The same with progress(submitted, completed, user) called from the
+ * calling thread after every submission (completed = requests already
+ * done) and, once all are submitted, after each request it has waited
+ * for (completed = requests waited for). */
+typedef void (*tsg_replay_progress)(int submitted, int completed, void *user);
+int tsg_server_replay_progress(tsg_server *s, int n, const int *handles, const int *ops, const double *arrival_ms,
+                               int keep, unsigned long long *ids, tsg_request_info *infos,
+                               tsg_replay_progress progress, void *user);
 
 /* Library build information (compile flags, target). */
 const char *tsg_build_info(void);

@@ -15,6 +15,8 @@
  *                       order then B-row order; the contraction the
  *                       reference build applies with -mfma, src/Makefile:29,
  *                       is made explicit as fma(a, b, acc))
+ *   ora_spgemm_rowlist  the same, for a list of rows (sampled-row checks of
+ *                       products too large for the whole oracle)
  *   ora_nnzcub          src/main.cu:391-397
  *   ora_csr2tile        src/csr2tile.h:6-259 (row-major tiles; in-tile
  *                       columns plain or packed (row<<4)|col)
@@ -171,6 +173,54 @@ long long ora_spgemm_rows(int r0, int r1, const int *rowptrA, const int *colA, c
     }
     free(seen); free(acc); free(list);
     return out;
+}
+
+/* Rows rows[0..nrows) of C = A*B, the same per-row restatement as
+ * ora_spgemm_rows (compute_dense_row order, src/external/cusparse/
+ * spgemm_serialref_spa.h:18-29), the rows spread over OpenMP threads.
+ * colC == NULL: per-row counts into cnt[x] (return = total).  Otherwise
+ * rowptrC[x] (from a count pass) places row x's sorted columns/values. */
+long long ora_spgemm_rowlist(int nrows, const int *rows, const int *rowptrA, const int *colA, const double *valA,
+                             const int *rowptrB, const int *colB, const double *valB, int nB, long long *cnt,
+                             const long long *rowptrC, int *colC, double *valC) {
+    long long total = 0;
+    int fail = 0;
+#pragma omp parallel reduction(+ : total) reduction(| : fail)
+    {
+        unsigned char *seen = (unsigned char *)calloc((size_t)nB + 1, 1);
+        double *acc = (double *)calloc((size_t)nB + 1, sizeof(double));
+        int *list = (int *)malloc(((size_t)nB + 1) * sizeof(int));
+        if (!seen || !acc || !list) {
+            fail = 1;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int x = 0; x < nrows; x++) {
+                const int i = rows[x];
+                int nl = 0;
+                for (int p = rowptrA[i]; p < rowptrA[i + 1]; p++) {
+                    const int k = colA[p];
+                    const double a = valA ? valA[p] : 0.0;
+                    for (int q = rowptrB[k]; q < rowptrB[k + 1]; q++) {
+                        const int j = colB[q];
+                        if (!seen[j]) { seen[j] = 1; list[nl++] = j; }
+                        if (valA && colC) acc[j] = fma(a, valB[q], acc[j]);
+                    }
+                }
+                if (colC) qsort(list, nl, sizeof(int), cmp_int);
+                for (int y = 0; y < nl; y++) {
+                    const int j = list[y];
+                    if (colC) colC[rowptrC[x] + y] = j;
+                    if (colC && valC) valC[rowptrC[x] + y] = acc[j];
+                    acc[j] = 0.0;
+                    seen[j] = 0;
+                }
+                if (!colC) cnt[x] = nl;
+                total += nl;
+            }
+        }
+        free(seen); free(acc); free(list);
+    }
+    return fail ? -1 : total;
 }
 
 unsigned long long ora_nnzcub(int nnzA, const int *colA, const int *rowptrB) {

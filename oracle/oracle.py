@@ -41,6 +41,7 @@ for name, res, args in [
     ("ora_spa_structure", _ll, [_i, _p, _p, _p, _p, _i, _p, _p, _i]),
     ("ora_spa_values", _i, [_i, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p]),
     ("ora_spgemm_rows", _ll, [_i, _i, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p]),
+    ("ora_spgemm_rowlist", _ll, [_i, _p, _p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p]),
     ("ora_nnzcub", C.c_ulonglong, [_i, _p, _p]),
     ("ora_csr2tile", _ll, [_i, _i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]),
     ("ora_step1_tiles", _ll, [_i, _p, _p, _i, _p, _p, _p, _p]),
@@ -104,6 +105,23 @@ def spgemm_rows(a, b, r0, r1, values=True):
     vv = np.zeros(max(nnz, 1), np.float64)
     lib.ora_spgemm_rows(r0, r1, *args, _ptr(rp), _ptr(cc), _ptr(vv) if values else None)
     return rp, cc[:nnz], vv[:nnz]
+
+
+def spgemm_rowlist(a, b, rows, values=True):
+    """Structure (+values) of the rows `rows` of C = A*B (any order):
+    returns (rowptr int64[len(rows)+1], columns, values)."""
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    cnt = np.zeros(len(r), np.int64)
+    args = (_ptr(_i32(a.rowptr)), _ptr(_i32(a.colidx)), _ptr(_f64(a.val)) if values else None,
+            _ptr(_i32(b.rowptr)), _ptr(_i32(b.colidx)), _ptr(_f64(b.val)), b.n)
+    tot = lib.ora_spgemm_rowlist(len(r), _ptr(r), *args, _ptr(cnt), None, None, None)
+    assert tot >= 0, "oracle allocation failed"
+    rp = np.zeros(len(r) + 1, np.int64)
+    np.cumsum(cnt, out=rp[1:])
+    cc = np.zeros(max(tot, 1), np.int32)
+    vv = np.zeros(max(tot, 1), np.float64)
+    lib.ora_spgemm_rowlist(len(r), _ptr(r), *args, None, _ptr(rp), _ptr(cc), _ptr(vv) if values else None)
+    return rp, cc[:tot], vv[:tot]
 
 
 def num_threads():

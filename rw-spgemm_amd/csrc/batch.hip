@@ -515,6 +515,18 @@ extern "C" int tsg_batch_result(tsg_batch *b, int i, tsg_htiles *tiles, tsg_hcsr
     return TSG_OK;
 }
 
+extern "C" int tsg_batch_result_checksum(tsg_batch *b, int i, tsg_csum *out) {
+    if (!b || i < 0 || i >= b->count) {
+        tsg_set_error("tsg_batch_result_checksum: bad problem index");
+        return TSG_ERR_ARG;
+    }
+    TSG_HIP(hipStreamSynchronize(b->ctx->stream));
+    tsg_dtiles v;
+    int jbase;
+    TSG_TRY(problem_view(b, i, &v, &jbase));
+    return tsg_tiles_checksum(b->ctx, &v, 0, jbase, out);
+}
+
 // ------------------------------------------------------------------ tile packing
 // Block-diagonal packing of resident tile matrices (no CSR round trip):
 // part p lands at tile-row TR_p, tile-column TC_p, tile TB_p and nonzero

@@ -146,11 +146,12 @@ k_c2t_finalize(int numtile, unsigned char *__restrict__ ptr16, int *__restrict__
 }
 
 // Input contract of the conversion: row pointers non-decreasing from 0 to
-// nnz, columns in [0, n) and strictly increasing inside each row.  Any
-// violation sets a flag instead of letting a later kernel index out of range.
+// nnz, columns in [0, n) and (when `sorted`) strictly increasing inside each
+// row.  Any violation sets a flag instead of letting a later kernel index
+// out of range.
 __global__ void __launch_bounds__(256)
 k_validate_csr(const int *__restrict__ rowptr, const int *__restrict__ colidx, int m, int n, int nnz,
-               int *__restrict__ bad) {
+               int sorted, int *__restrict__ bad) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i > m) return;
     const int s = rowptr[i];
@@ -164,27 +165,30 @@ k_validate_csr(const int *__restrict__ rowptr, const int *__restrict__ colidx, i
     int prev = -1;
     for (int k = s; k < e; k++) {
         const int c = colidx[k];
-        if (c <= prev || c >= n) { atomicOr(bad, 2); return; }
+        if ((sorted && c <= prev) || c < 0 || c >= n) { atomicOr(bad, 2); return; }
         prev = c;
     }
 }
 
-int tsg_validate_csr(tsg_ctx *ctx, const tsg_dcsr *a) {
+int tsg_validate_csr_mode(tsg_ctx *ctx, const tsg_dcsr *a, int sorted) {
     void *scr;
     TSG_TRY(tsg_scratch(ctx, 64, &scr));
     TSG_HIP(hipMemsetAsync(scr, 0, 4, ctx->stream));
     hipLaunchKernelGGL(k_validate_csr, dim3(tsg_div_up((long long)a->m + 1, 256)), dim3(256), 0, ctx->stream,
-                       a->rowptr, a->colidx, a->m, a->n, a->nnz, (int *)scr);
+                       a->rowptr, a->colidx, a->m, a->n, a->nnz, sorted, (int *)scr);
     TSG_LAUNCH_CHECK();
     long long bad = 0;
     TSG_TRY(tsg_read_scalar_i64(ctx, scr, 0, &bad));
     if (bad) {
         tsg_set_error(bad & 1 ? "CSR row pointers inconsistent with nnz"
-                              : "CSR columns out of range or not strictly increasing inside a row");
+                      : sorted ? "CSR columns out of range or not strictly increasing inside a row"
+                               : "CSR columns out of range");
         return TSG_ERR_ARG;
     }
     return TSG_OK;
 }
+
+int tsg_validate_csr(tsg_ctx *ctx, const tsg_dcsr *a) { return tsg_validate_csr_mode(ctx, a, 1); }
 
 extern "C" int tsg_csr2tile(tsg_ctx *ctx, const tsg_dcsr *a, tsg_dtiles *out) {
     memset(out, 0, sizeof(*out));
@@ -374,6 +378,202 @@ int tsg_tile2csr_view(tsg_ctx *ctx, const tsg_dtiles *t, int jbase, tsg_dcsr *ou
         TSG_LAUNCH_CHECK();
     }
     return TSG_OK;
+}
+
+// ------------------------------------------------------------------ checksums, sampled rows
+// splitmix64's finalizer: a bijective 64-bit mix.
+__device__ __forceinline__ unsigned long long csum_mix(unsigned long long x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// One thread per row of the view: the row's entries (as tile2csr emits
+// them) and its share of every tile of the tile-row (row mask and row
+// offset; row 0 adds the tile's nonzero count).  Sums are order-free, so
+// they add up over chunks.  acc[0] tiles, acc[1] entries, acc[2] nonzeros,
+// acc[3] tiles counted, acc[4] entries with row and column swapped (equal
+// to acc[1] exactly when C is bit-symmetric).
+__global__ void __launch_bounds__(256)
+k_checksum(int m, const int *__restrict__ tile_ptr, const int *__restrict__ tile_colidx,
+           const int *__restrict__ tile_nnz, const unsigned char *__restrict__ csr_ptr,
+           const unsigned short *__restrict__ mask, const unsigned char *__restrict__ tcol,
+           const double *__restrict__ tval, long long row_base, int jbase, unsigned long long *__restrict__ acc) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long ht = 0, he = 0, hx = 0, nz = 0, nt = 0;
+    if (i < m) {
+        const int I = i >> 4, r = i & 15;
+        const int rowlen = min(16, m - I * 16);
+        const long long row = row_base + i, Ig = (row_base >> 4) + I;
+        for (int t = tile_ptr[I]; t < tile_ptr[I + 1]; t++) {
+            const long long J = tile_colidx[t] - jbase;
+            const unsigned long long tk = csum_mix(((unsigned long long)Ig << 32) ^ (unsigned long long)J);
+            ht += csum_mix(tk ^ ((unsigned long long)(r + 1) << 40) ^
+                           ((unsigned long long)mask[(long long)t * 16 + r] << 8) ^ csr_ptr[(long long)t * 16 + r]);
+            if (r == 0) {
+                ht += csum_mix(tk + (unsigned long long)(tile_nnz[t + 1] - tile_nnz[t]));
+                nt++;
+            }
+            const int k = t2c_row_len(tile_nnz, csr_ptr, t, r, rowlen);
+            const int s = tile_nnz[t] + csr_ptr[(long long)t * 16 + r];
+            for (int j = 0; j < k; j++) {
+                const unsigned long long col = (unsigned long long)(J * 16 + tcol[s + j]);
+                const unsigned long long vb = (unsigned long long)__double_as_longlong(tval[s + j]);
+                he += csum_mix(csum_mix(((unsigned long long)row << 32) ^ col) ^ vb);
+                hx += csum_mix(csum_mix((col << 32) ^ (unsigned long long)row) ^ vb);
+            }
+            nz += k > 0 ? k : 0;
+        }
+    }
+    ht = tsg_wave_sum(ht);
+    he = tsg_wave_sum(he);
+    hx = tsg_wave_sum(hx);
+    nz = tsg_wave_sum(nz);
+    nt = tsg_wave_sum(nt);
+    if (__lane_id() == 0) {
+        atomicAdd(&acc[0], ht);
+        atomicAdd(&acc[1], he);
+        atomicAdd(&acc[2], nz);
+        atomicAdd(&acc[3], nt);
+        atomicAdd(&acc[4], hx);
+    }
+}
+
+extern "C" int tsg_tiles_checksum(tsg_ctx *ctx, const tsg_dtiles *c, long long row_base, int jbase,
+                                  tsg_csum *out) {
+    memset(out, 0, sizeof(*out));
+    if (row_base < 0 || (row_base & 15)) {
+        tsg_set_error("tsg_tiles_checksum: row_base must be a non-negative multiple of 16");
+        return TSG_ERR_ARG;
+    }
+    if (c->m <= 0) return TSG_OK;
+    unsigned long long *acc;
+    TSG_TRY(tsg_alloc_t(ctx, &acc, 5));
+    int st = TSG_OK;
+    if (hipMemsetAsync(acc, 0, 40, ctx->stream) != hipSuccess) {
+        tsg_set_error("tsg_tiles_checksum: memset failed");
+        st = TSG_ERR_HIP;
+    }
+    if (st == TSG_OK) {
+        hipLaunchKernelGGL(k_checksum, dim3(tsg_div_up(c->m, 256)), dim3(256), 0, ctx->stream, c->m, c->tile_ptr,
+                           c->tile_colidx, c->tile_nnz, c->csr_ptr, c->mask, c->csr_col, c->val, row_base, jbase,
+                           acc);
+        if (hipGetLastError() != hipSuccess) {
+            tsg_set_error("tsg_tiles_checksum: launch failed");
+            st = TSG_ERR_HIP;
+        }
+    }
+    unsigned long long h[5] = {0, 0, 0, 0, 0};
+    if (st == TSG_OK) st = tsg_d2h(ctx, h, acc, 40);
+    if (st == TSG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        tsg_set_error("tsg_tiles_checksum: synchronize failed");
+        st = TSG_ERR_HIP;
+    }
+    tsg_free(ctx, acc);
+    TSG_TRY(st);
+    out->tiles = h[0];
+    out->entries = h[1];
+    out->nnz = (long long)h[2];
+    out->numtile = (long long)h[3];
+    out->entries_t = h[4];
+    return TSG_OK;
+}
+
+// Selected rows of a view (row ids local to the view, any order) as CSR.
+template <int PASS>
+__global__ void __launch_bounds__(256)
+k_rows_sel(int m, int nsel, const int *__restrict__ sel, const int *__restrict__ tile_ptr,
+           const int *__restrict__ tile_colidx, const int *__restrict__ tile_nnz,
+           const unsigned char *__restrict__ csr_ptr, const unsigned char *__restrict__ tcol,
+           const double *__restrict__ tval, int *__restrict__ cnt_or_ptr, int *__restrict__ col,
+           double *__restrict__ val, int jbase) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= nsel) return;
+    const int i = sel[x];
+    const int I = i >> 4, r = i & 15;
+    const int rowlen = min(16, m - I * 16);
+    const int t0 = tile_ptr[I], t1 = tile_ptr[I + 1];
+    if (PASS == 0) {
+        int c = 0;
+        for (int t = t0; t < t1; t++) c += t2c_row_len(tile_nnz, csr_ptr, t, r, rowlen);
+        cnt_or_ptr[x] = c;
+    } else {
+        int out = cnt_or_ptr[x];
+        for (int t = t0; t < t1; t++) {
+            const int k = t2c_row_len(tile_nnz, csr_ptr, t, r, rowlen);
+            if (k <= 0) continue;
+            const int s = tile_nnz[t] + csr_ptr[(long long)t * 16 + r];
+            const int J16 = (tile_colidx[t] - jbase) * 16;
+            for (int j = 0; j < k; j++) {
+                col[out + j] = J16 + tcol[s + j];
+                val[out + j] = tval[s + j];
+            }
+            out += k;
+        }
+    }
+}
+
+extern "C" int tsg_tiles_rows(tsg_ctx *ctx, const tsg_dtiles *c, int jbase, const int *rows, int nrows,
+                              tsg_hcsr *out) {
+    memset(out, 0, sizeof(*out));
+    if (nrows < 0 || (nrows > 0 && !rows)) { tsg_set_error("tsg_tiles_rows: bad row list"); return TSG_ERR_ARG; }
+    for (int x = 0; x < nrows; x++)
+        if (rows[x] < 0 || rows[x] >= c->m) {
+            tsg_set_error("tsg_tiles_rows: row %d outside [0, %d)", rows[x], c->m);
+            return TSG_ERR_ARG;
+        }
+    out->m = nrows;
+    out->n = c->n;
+    out->rowptr = (int *)malloc(4 * ((size_t)nrows + 1));
+    if (!out->rowptr) { tsg_set_error("tsg_tiles_rows: host allocation failed"); return TSG_ERR_NOMEM; }
+    out->rowptr[0] = 0;
+    if (nrows == 0) {
+        out->colidx = (int *)malloc(4);
+        out->val = (double *)malloc(8);
+        return TSG_OK;
+    }
+    int *dsel = nullptr, *dptr = nullptr, *dcol = nullptr;
+    double *dval = nullptr;
+    long long nnz = 0;
+    const dim3 grid(tsg_div_up(nrows, 256));
+    int st = tsg_alloc_t(ctx, &dsel, (size_t)nrows);
+    if (st == TSG_OK) st = tsg_alloc_t(ctx, &dptr, (size_t)nrows + 1);
+    if (st == TSG_OK) st = tsg_h2d(ctx, dsel, rows, 4 * (size_t)nrows);
+    if (st == TSG_OK) {
+        hipLaunchKernelGGL(k_rows_sel<0>, grid, dim3(256), 0, ctx->stream, c->m, nrows, dsel, c->tile_ptr,
+                           c->tile_colidx, c->tile_nnz, c->csr_ptr, c->csr_col, c->val, dptr, (int *)nullptr,
+                           (double *)nullptr, jbase);
+        if (hipGetLastError() != hipSuccess) { tsg_set_error("tsg_tiles_rows: launch failed"); st = TSG_ERR_HIP; }
+    }
+    if (st == TSG_OK) st = tsg_scan_i32(ctx, dptr, dptr, nrows, &nnz);
+    if (st == TSG_OK) st = tsg_alloc_t(ctx, &dcol, (size_t)nnz + 1);
+    if (st == TSG_OK) st = tsg_alloc_t(ctx, &dval, (size_t)nnz + 1);
+    if (st == TSG_OK && nnz > 0) {
+        hipLaunchKernelGGL(k_rows_sel<1>, grid, dim3(256), 0, ctx->stream, c->m, nrows, dsel, c->tile_ptr,
+                           c->tile_colidx, c->tile_nnz, c->csr_ptr, c->csr_col, c->val, dptr, dcol, dval, jbase);
+        if (hipGetLastError() != hipSuccess) { tsg_set_error("tsg_tiles_rows: launch failed"); st = TSG_ERR_HIP; }
+    }
+    if (st == TSG_OK) {
+        out->nnz = (int)nnz;
+        out->colidx = (int *)malloc(4 * (size_t)(nnz ? nnz : 1));
+        out->val = (double *)malloc(8 * (size_t)(nnz ? nnz : 1));
+        if (!out->colidx || !out->val) { tsg_set_error("tsg_tiles_rows: host allocation failed"); st = TSG_ERR_NOMEM; }
+    }
+    if (st == TSG_OK) st = tsg_d2h(ctx, out->rowptr, dptr, 4 * ((size_t)nrows + 1));
+    if (st == TSG_OK && nnz > 0) st = tsg_d2h(ctx, out->colidx, dcol, 4 * (size_t)nnz);
+    if (st == TSG_OK && nnz > 0) st = tsg_d2h(ctx, out->val, dval, 8 * (size_t)nnz);
+    if (st == TSG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        tsg_set_error("tsg_tiles_rows: synchronize failed");
+        st = TSG_ERR_HIP;
+    }
+    tsg_free(ctx, dval);
+    tsg_free(ctx, dcol);
+    tsg_free(ctx, dptr);
+    tsg_free(ctx, dsel);
+    if (st != TSG_OK) tsg_hcsr_free(out);
+    return st;
 }
 
 extern "C" int tsg_tiles_build_rowindex(tsg_ctx *ctx, tsg_dtiles *t) {

@@ -120,7 +120,10 @@ static int sort_ids_by_degree(tsg_ctx *ctx, const int *deg, int n, int descendin
 
 extern "C" int tsg_order_degree(tsg_ctx *ctx, const tsg_dcsr *a, int symmetric_header, int *d_old_id) {
     if (a->m != a->n) { tsg_set_error("degree order needs a square matrix"); return TSG_ERR_ARG; }
-    TSG_TRY(tsg_validate_csr(ctx, a));
+    // only row lengths (and, for the mirror count, column ids) are read:
+    // a CSR straight from mmio_allinone, repeated entries included, is
+    // ranked by its row lengths as given
+    TSG_TRY(tsg_validate_csr_mode(ctx, a, 0));
     int *deg;
     TSG_TRY(tsg_alloc_t(ctx, &deg, (size_t)a->m + 1));
     const unsigned g = tsg_div_up(a->m, 256);

@@ -189,7 +189,10 @@ extern "C" int tsg_server_create(int device, const tsg_server_config *cfg, tsg_s
     return TSG_OK;
 }
 
-// Runs every chunk of A*B once on ctx to size it exactly.
+// Sizes A*B exactly from the symbolic steps: every chunk runs steps 1-2 and
+// the numeric step's plan (row classes, heavy-row work list), not the
+// numeric step itself.  C's tiles and nonzeros per chunk, its element
+// segments and heavy-row scratch give the request's admission bytes.
 static int size_product(tsg_server *s, tsg_ctx *ctx, const tsg_dtiles &A, const tsg_dtiles &B, OpSize &os) {
     int nb_max = 1 << 16;
     std::vector<int> bounds(nb_max + 1);
@@ -202,7 +205,7 @@ static int size_product(tsg_server *s, tsg_ctx *ctx, const tsg_dtiles &A, const 
     for (int c = 0; c < n; c++) {
         tsg_dtiles C;
         tsg_stats st;
-        TSG_TRY(tsg_spgemm_tile_rows(ctx, &A, &B, bounds[c], bounds[c + 1], &s->cfg.tune, &C, &st));
+        TSG_TRY(tsg_spgemm_tile_rows_ex(ctx, &A, &B, bounds[c], bounds[c + 1], &s->cfg.tune, &C, &st, true));
         os.numtileC += st.numtileC;
         os.nnzC += st.nnzC;
         os.bytes += c_bytes(C.tilem, st.numtileC, st.nnzC, tsg_s1_words(&B, s->cfg.tune));
@@ -577,41 +580,142 @@ extern "C" int tsg_server_release(tsg_server *s, unsigned long long id) {
     return TSG_OK;
 }
 
+// The C views of a finished, unreleased request: its chunks (first tile-row
+// of each in r0s) or its share of a fused product (one view, tile columns
+// relative to jb).
+static int result_views(tsg_server *s, unsigned long long id, const char *who, std::vector<tsg_dtiles> &views,
+                        std::vector<int> &jb, std::vector<int> &r0s, int *m_out, int *n_out) {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { tsg_set_error("%s: unknown request", who); return TSG_ERR_ARG; }
+    Request *r = it->second.get();
+    if (!r->done || r->released || r->status) {
+        tsg_set_error("%s: request %llu has no result", who, id);
+        return TSG_ERR_STATE;
+    }
+    const Matrix &M = *s->mats[r->handle];
+    const int m = M.a.m;
+    const int n = r->op == TSG_OP_A2 ? M.a.n : M.a.m;
+    *m_out = m;
+    *n_out = n;
+    if (r->fused) {
+        tsg_dtiles v = r->fused->C;
+        v.tile_ptr = r->fused->C.tile_ptr + r->f_tr;
+        v.m = m;
+        v.tilem = (m + 15) / 16;
+        v.n = n;
+        v.tilen = (n + 15) / 16;
+        views.push_back(v);
+        jb.push_back(r->f_tc);
+        r0s.push_back(0);
+    } else {
+        for (size_t c = 0; c < r->chunks.size(); c++) {
+            views.push_back(r->chunks[c]);
+            jb.push_back(0);
+            r0s.push_back(r->chunk_r0[c]);
+        }
+    }
+    return TSG_OK;
+}
+
+extern "C" int tsg_server_result_checksum(tsg_server *s, unsigned long long id, tsg_csum *out) {
+    memset(out, 0, sizeof(*out));
+    std::vector<tsg_dtiles> views;
+    std::vector<int> jb, r0s;
+    int m = 0, n = 0;
+    TSG_TRY(result_views(s, id, "tsg_server_result_checksum", views, jb, r0s, &m, &n));
+    std::lock_guard<std::mutex> g(s->main_mu);
+    for (size_t i = 0; i < views.size(); i++) {
+        tsg_csum c;
+        TSG_TRY(tsg_tiles_checksum(s->main, &views[i], 16LL * r0s[i], jb[i], &c));
+        out->numtile += c.numtile;
+        out->nnz += c.nnz;
+        out->tiles += c.tiles;
+        out->entries += c.entries;
+        out->entries_t += c.entries_t;
+    }
+    return TSG_OK;
+}
+
+extern "C" int tsg_server_result_rows(tsg_server *s, unsigned long long id, const int *rows, int nrows,
+                                      tsg_hcsr *out) {
+    memset(out, 0, sizeof(*out));
+    if (nrows < 0 || (nrows > 0 && !rows)) { tsg_set_error("tsg_server_result_rows: bad row list"); return TSG_ERR_ARG; }
+    std::vector<tsg_dtiles> views;
+    std::vector<int> jb, r0s;
+    int m = 0, n = 0;
+    TSG_TRY(result_views(s, id, "tsg_server_result_rows", views, jb, r0s, &m, &n));
+    for (int x = 0; x < nrows; x++)
+        if (rows[x] < 0 || rows[x] >= m) {
+            tsg_set_error("tsg_server_result_rows: row %d outside [0, %d)", rows[x], m);
+            return TSG_ERR_ARG;
+        }
+    // rows of each view, then reassembled in the caller's order
+    std::vector<std::vector<int>> local(views.size()), pos(views.size());
+    for (int x = 0; x < nrows; x++) {
+        size_t v = views.size() - 1;
+        while (v > 0 && 16 * r0s[v] > rows[x]) v--;
+        local[v].push_back(rows[x] - 16 * r0s[v]);
+        pos[v].push_back(x);
+    }
+    std::vector<tsg_hcsr> parts(views.size());
+    std::vector<int> len(nrows, 0), from_part(nrows, 0), from_row(nrows, 0);
+    int st = TSG_OK;
+    {
+        std::lock_guard<std::mutex> g(s->main_mu);
+        for (size_t v = 0; v < views.size() && st == TSG_OK; v++) {
+            memset(&parts[v], 0, sizeof(parts[v]));
+            if (local[v].empty()) continue;
+            st = tsg_tiles_rows(s->main, &views[v], jb[v], local[v].data(), (int)local[v].size(), &parts[v]);
+        }
+        tsg_ctx_trim(s->main);
+    }
+    long long nnz = 0;
+    if (st == TSG_OK) {
+        for (size_t v = 0; v < views.size(); v++)
+            for (size_t k = 0; k < pos[v].size(); k++) {
+                const int x = pos[v][k];
+                len[x] = parts[v].rowptr[k + 1] - parts[v].rowptr[k];
+                from_part[x] = (int)v;
+                from_row[x] = (int)k;
+                nnz += len[x];
+            }
+        out->m = nrows;
+        out->n = n;
+        out->nnz = (int)nnz;
+        out->rowptr = (int *)malloc(4 * ((size_t)nrows + 1));
+        out->colidx = (int *)malloc(4 * (size_t)(nnz ? nnz : 1));
+        out->val = (double *)malloc(8 * (size_t)(nnz ? nnz : 1));
+        if (!out->rowptr || !out->colidx || !out->val) {
+            tsg_set_error("tsg_server_result_rows: host allocation failed");
+            st = TSG_ERR_NOMEM;
+        }
+    }
+    if (st == TSG_OK) {
+        long long e = 0;
+        for (int x = 0; x < nrows; x++) {
+            out->rowptr[x] = (int)e;
+            const tsg_hcsr &p = parts[from_part[x]];
+            const int s0 = p.rowptr[from_row[x]];
+            memcpy(out->colidx + e, p.colidx + s0, 4 * (size_t)len[x]);
+            memcpy(out->val + e, p.val + s0, 8 * (size_t)len[x]);
+            e += len[x];
+        }
+        out->rowptr[nrows] = (int)e;
+    }
+    for (auto &p : parts) tsg_hcsr_free(&p);
+    if (st != TSG_OK) tsg_hcsr_free(out);
+    return st;
+}
+
 // C of a finished, unreleased request as a host CSR (rows of every chunk
 // concatenated).
 extern "C" int tsg_server_result_csr(tsg_server *s, unsigned long long id, tsg_hcsr *out) {
     memset(out, 0, sizeof(*out));
     std::vector<tsg_dtiles> views;
-    std::vector<int> jb;
+    std::vector<int> jb, r0s;
     int m = 0, n = 0;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        auto it = s->reqs.find(id);
-        if (it == s->reqs.end()) { tsg_set_error("tsg_server_result_csr: unknown request"); return TSG_ERR_ARG; }
-        Request *r = it->second.get();
-        if (!r->done || r->released || r->status) {
-            tsg_set_error("tsg_server_result_csr: request %llu has no result", id);
-            return TSG_ERR_STATE;
-        }
-        const Matrix &M = *s->mats[r->handle];
-        m = M.a.m;
-        n = r->op == TSG_OP_A2 ? M.a.n : M.a.m;
-        if (r->fused) {
-            tsg_dtiles v = r->fused->C;
-            v.tile_ptr = r->fused->C.tile_ptr + r->f_tr;
-            v.m = m;
-            v.tilem = (m + 15) / 16;
-            v.n = n;
-            v.tilen = (n + 15) / 16;
-            views.push_back(v);
-            jb.push_back(r->f_tc);
-        } else {
-            for (auto &c : r->chunks) {
-                views.push_back(c);
-                jb.push_back(0);
-            }
-        }
-    }
+    TSG_TRY(result_views(s, id, "tsg_server_result_csr", views, jb, r0s, &m, &n));
     std::lock_guard<std::mutex> g(s->main_mu);
     tsg_ctx *ctx = s->main;
     std::vector<tsg_hcsr> parts(views.size());
@@ -670,6 +774,12 @@ extern "C" int tsg_server_stats(tsg_server *s, tsg_server_stats_t *out) {
 // budget); keep = 1 leaves them for tsg_server_result_csr / release.
 extern "C" int tsg_server_replay(tsg_server *s, int n, const int *handles, const int *ops, const double *arrival_ms,
                                  int keep, unsigned long long *ids, tsg_request_info *infos) {
+    return tsg_server_replay_progress(s, n, handles, ops, arrival_ms, keep, ids, infos, nullptr, nullptr);
+}
+
+extern "C" int tsg_server_replay_progress(tsg_server *s, int n, const int *handles, const int *ops,
+                                          const double *arrival_ms, int keep, unsigned long long *ids,
+                                          tsg_request_info *infos, tsg_replay_progress progress, void *user) {
     const double t0 = clock_ms();
     int first_err = TSG_OK;
     for (int i = 0; i < n; i++) {
@@ -679,18 +789,29 @@ extern "C" int tsg_server_replay(tsg_server *s, int n, const int *handles, const
             const double d = due - now;
             if (d > 0.2) std::this_thread::sleep_for(std::chrono::microseconds((long long)((d - 0.1) * 1000)));
         }
-        std::lock_guard<std::mutex> g(s->mu);
-        const int st = submit_locked(s, handles[i], ops[i], !keep, &ids[i]);
-        if (st) { ids[i] = 0; if (!first_err) first_err = st; }
+        int done = 0;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            const int st = submit_locked(s, handles[i], ops[i], !keep, &ids[i]);
+            if (st) { ids[i] = 0; if (!first_err) first_err = st; }
+            if (progress)
+                for (int j = 0; j <= i; j++) {
+                    auto it = ids[j] ? s->reqs.find(ids[j]) : s->reqs.end();
+                    done += it != s->reqs.end() && it->second->done;
+                }
+        }
+        if (progress) progress(i + 1, done, user);
     }
     for (int i = 0; i < n; i++) {
-        if (!ids[i]) continue;
-        const int st = tsg_server_wait(s, ids[i], 0, infos ? &infos[i] : nullptr);
-        if (st && !first_err) first_err = st;
-        if (!keep) {
-            std::lock_guard<std::mutex> g(s->mu);
-            s->reqs.erase(ids[i]);
+        if (ids[i]) {
+            const int st = tsg_server_wait(s, ids[i], 0, infos ? &infos[i] : nullptr);
+            if (st && !first_err) first_err = st;
+            if (!keep) {
+                std::lock_guard<std::mutex> g(s->mu);
+                s->reqs.erase(ids[i]);
+            }
         }
+        if (progress) progress(n, i + 1, user);
     }
     return first_err;
 }

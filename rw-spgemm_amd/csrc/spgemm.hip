@@ -2003,9 +2003,33 @@ static int alloc_c_tiles(tsg_ctx *ctx, tsg_dtiles *C, long long numtileC) {
     return TSG_OK;
 }
 
-int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
-                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out) {
-    memset(C, 0, sizeof(*C));
+// Temporaries of one pipeline run, freed on every exit (stream-ordered: the
+// caching allocator hands a block out again only behind the work queued on
+// the context stream).
+struct SpgemmTemps {
+    unsigned *keep = nullptr;
+    void *scr = nullptr, *wbuf = nullptr, *w3buf = nullptr;
+    unsigned char *hflag = nullptr;
+    int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr;
+    int2 *hitems = nullptr;
+    hipEvent_t ev[6] = {};
+    int nev = 0;
+};
+
+static void free_temps(tsg_ctx *ctx, SpgemmTemps &t) {
+    void *ps[] = {t.keep, t.scr, t.wbuf, t.w3buf, t.hflag, t.hcnt, t.hinfo, t.hcur, t.hitems};
+    for (void *p : ps) tsg_free(ctx, p);
+    for (int i = 0; i < t.nev; i++) (void)hipEventDestroy(t.ev[i]);
+    t = SpgemmTemps();
+}
+
+// symbolic: steps 1 and 2 and the step-3 plan (row classes, element
+// segments, heavy-row work list) only; C's values and in-tile columns are
+// neither allocated nor computed.  stats then carries C's exact tile and
+// nonzero counts and the heavy-row scratch the numeric step would add.
+static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
+                      tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out, bool symbolic,
+                      SpgemmTemps &T) {
     if (!A || !B || A->n != B->m) {
         tsg_set_error("tsg_spgemm: inner dimensions differ (%d vs %d)", A ? A->n : -1, B ? B->m : -1);
         return TSG_ERR_ARG;
@@ -2026,11 +2050,11 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     P.elem_all = B->rowx_ptr != nullptr && tune.element_mode == 2;
     hipStream_t s = ctx->stream;
 
-    hipEvent_t ev[6];
+    hipEvent_t *ev = T.ev;
     const bool timed = stats != nullptr;
     if (timed) {
         memset(stats, 0, sizeof(*stats));
-        for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
+        for (; T.nev < 6; T.nev++) TSG_HIP(hipEventCreate(&ev[T.nev]));
         TSG_HIP(hipEventRecord(ev[0], s));
         TSG_HIP(hipEventRecord(ev[4], s));
         TSG_HIP(hipEventRecord(ev[5], s));
@@ -2046,12 +2070,11 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         return TSG_ERR_ARG;
     }
     P.s1w = tsg_s1_words(B, tune);
-    unsigned *keep = nullptr;
     const size_t kb = keep_bytes(C->tilem, P.s1w);
-    if (kb) TSG_TRY(tsg_alloc(ctx, (void **)&keep, kb));
-    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, keep));
-    void *scr;
-    TSG_TRY(tsg_alloc(ctx, &scr, 64));
+    if (kb) TSG_TRY(tsg_alloc(ctx, (void **)&T.keep, kb));
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep));
+    TSG_TRY(tsg_alloc(ctx, &T.scr, 64));
+    void *scr = T.scr;
     unsigned long long *bins = (unsigned long long *)scr;
     TSG_HIP(hipMemsetAsync(scr, 0, 64, s));
     long long numtileC = 0;
@@ -2059,17 +2082,15 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     P.numtileC = numtileC;
     C->numtile = (int)numtileC;
     TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
-    void *wbuf = nullptr, *w3buf = nullptr;
-    unsigned char *hflag = nullptr;
-    int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr;
-    int2 *hitems = nullptr;
     size_t woff[WORK_NARR];
     const size_t wbytes = work_layout(C->tilem, numtileC, 0, woff);
-    TSG_TRY(tsg_alloc(ctx, &wbuf, woff[WORK_NARR - 1]));  // everything but the step-3 segments
+    TSG_TRY(tsg_alloc(ctx, &T.wbuf, woff[WORK_NARR - 1]));  // everything but the step-3 segments
     (void)wbytes;
+    void *wbuf = T.wbuf;
     SpgemmWork w = work_at(wbuf, C->tilem, numtileC, 0, nullptr);
-    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, keep));
-    tsg_free(ctx, keep);
+    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, T.keep));
+    tsg_free(ctx, T.keep);
+    T.keep = nullptr;
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
     // step-2 LDS sizing: the largest segment and the widest tile-row span
     P.s2u = G2_UMAX;
@@ -2100,8 +2121,10 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
     C->nnz = (int)nnzC;
     if (timed) TSG_HIP(hipEventRecord(ev[2], s));
     // ---- step 3
-    TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
-    TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
+    if (!symbolic) {
+        TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
+        TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
+    }
     if (nnzC > 0) {
         if (!P.all) {
             TSG_HIP(hipMemsetAsync(w.cls_out, 0, 8 * sizeof(int), s));
@@ -2115,15 +2138,17 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         }
         if (!P.all && P.s3_unfit && P.s3_elem) {
             const SpgemmWork w3 = work_at(wbuf, C->tilem, numtileC, nnzC, nullptr);
-            TSG_TRY(tsg_alloc(ctx, &w3buf, 16 * (size_t)w3.seg3_cap));
-            w.seg3 = (int4 *)w3buf;
+            TSG_TRY(tsg_alloc(ctx, &T.w3buf, 16 * (size_t)w3.seg3_cap));
+            w.seg3 = (int4 *)T.w3buf;
             w.seg3_cap = w3.seg3_cap;
             if (op.Arowp && !tune.no_heavy_rows) {
                 // rows with very many products: one wave each (k_step3_heavy)
                 const int m = C->m;
-                TSG_TRY(tsg_alloc_t(ctx, &hflag, (size_t)m + 16));
-                TSG_TRY(tsg_alloc_t(ctx, &hcnt, (size_t)m + 1));
-                TSG_TRY(tsg_alloc_t(ctx, &hinfo, 4));  // {longest heavy A row, heavy rows, item counter, pad}
+                TSG_TRY(tsg_alloc_t(ctx, &T.hflag, (size_t)m + 16));
+                TSG_TRY(tsg_alloc_t(ctx, &T.hcnt, (size_t)m + 1));
+                TSG_TRY(tsg_alloc_t(ctx, &T.hinfo, 4));  // {longest heavy A row, heavy rows, item counter, pad}
+                unsigned char *hflag = T.hflag;
+                int *hcnt = T.hcnt, *hinfo = T.hinfo;
                 TSG_HIP(hipMemsetAsync(hinfo, 0, 16, s));
                 hipLaunchKernelGGL(k_heavy_flags, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, op.Arowp, op.Acolx,
                                    (const int *)B->rowx_ptr, (const unsigned char *)w.rowcls, C->tile_ptr,
@@ -2144,7 +2169,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                     long long info = 0;  // {longest heavy A row, heavy rows}
                     TSG_TRY(tsg_read_scalar_i64(ctx, hinfo, 1, &info));
                     const long long maxdeg = info & 0xffffffffLL, nrows = info >> 32;
-                    TSG_TRY(tsg_alloc_t(ctx, &hitems, (size_t)ni));
+                    TSG_TRY(tsg_alloc_t(ctx, &T.hitems, (size_t)ni));
+                    int2 *hitems = T.hitems;
                     hipLaunchKernelGGL(k_heavy_list, dim3(tsg_div_up(m, 256)), dim3(256), 0, s, (const int *)hcnt,
                                        (const unsigned char *)hflag, C->tile_ptr, C->tile_colidx, m, hitems);
                     TSG_LAUNCH_CHECK();
@@ -2152,7 +2178,8 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                     const long long by_mem = std::max<long long>(256, HEAVY_CURSOR_BYTES / (4 * std::max(1LL, maxdeg)));
                     w.hwaves = (int)std::min<long long>(ni, std::min<long long>(HEAVY_MAX_WAVES, by_mem));
                     w.hstride = std::max(1LL, maxdeg);
-                    TSG_TRY(tsg_alloc_t(ctx, &hcur, (size_t)w.hwaves * (size_t)w.hstride));
+                    if (!symbolic) TSG_TRY(tsg_alloc_t(ctx, &T.hcur, (size_t)w.hwaves * (size_t)w.hstride));
+                    int *hcur = T.hcur;
                     w.hflag = hflag;
                     w.hitems = hitems;
                     w.hcount = hcnt + m;
@@ -2163,7 +2190,7 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
                 }
             }
         }
-        TSG_TRY(launch_step3(s, op, B, C, P, w, timed ? ev + 4 : nullptr));
+        if (!symbolic) TSG_TRY(launch_step3(s, op, B, C, P, w, timed ? ev + 4 : nullptr));
     }
     if (timed) {
         TSG_HIP(hipEventRecord(ev[3], s));
@@ -2183,19 +2210,21 @@ int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, c
         stats->nnzC = nnzC;
         stats->heavy_rows = w.nheavy;
         stats->heavy_bytes = (long long)w.hwaves * w.hstride * 4 + (w.nheavy ? 5LL * C->m + 8LL * w.nitems : 0);
-        for (auto &e : ev) hipEventDestroy(e);
     }
-    tsg_free(ctx, hcur);
-    tsg_free(ctx, hitems);
-    tsg_free(ctx, hinfo);
-    tsg_free(ctx, hcnt);
-    tsg_free(ctx, hflag);
-    tsg_free(ctx, w3buf);
-    tsg_free(ctx, wbuf);
-    tsg_free(ctx, scr);
     if (plan_out) *plan_out = P;
     return TSG_OK;
 }
+
+int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
+                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out, bool symbolic) {
+    memset(C, 0, sizeof(*C));
+    SpgemmTemps T;
+    const int st = spgemm_run(ctx, A, B, tune_in, C, stats, plan_out, symbolic, T);
+    free_temps(ctx, T);
+    if (st != TSG_OK) tsg_tiles_free(ctx, C);  // a partly built C goes back too
+    return st;
+}
+
 
 extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune_in,
                           tsg_dtiles *C, tsg_stats *stats) {
@@ -2325,7 +2354,13 @@ extern "C" int tsg_spgemm_chunk_plan(tsg_ctx *ctx, const tsg_dtiles *A, const ts
 // as a view (tile_ptr offset into A's, every other array shared).
 extern "C" int tsg_spgemm_tile_rows(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, int r0, int r1,
                                     const tsg_tune *tune, tsg_dtiles *C, tsg_stats *stats) {
+    return tsg_spgemm_tile_rows_ex(ctx, A, B, r0, r1, tune, C, stats, false);
+}
+
+int tsg_spgemm_tile_rows_ex(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, int r0, int r1,
+                            const tsg_tune *tune, tsg_dtiles *C, tsg_stats *stats, bool symbolic) {
     if (!A || r0 < 0 || r1 < r0 || r1 > A->tilem) {
+        memset(C, 0, sizeof(*C));
         tsg_set_error("tsg_spgemm_tile_rows: bad tile-row range [%d, %d)", r0, r1);
         return TSG_ERR_ARG;
     }
@@ -2335,5 +2370,5 @@ extern "C" int tsg_spgemm_tile_rows(tsg_ctx *ctx, const tsg_dtiles *A, const tsg
     v.tilem = r1 - r0;
     v.m = std::min(16 * r1, A->m) - 16 * r0;
     if (v.m < 0) v.m = 0;
-    return tsg_spgemm_planned(ctx, &v, B, tune, C, stats, nullptr);
+    return tsg_spgemm_planned(ctx, &v, B, tune, C, stats, nullptr, symbolic);
 }

@@ -135,8 +135,16 @@ struct tsg_spgemm_plan {
     int s2map;      // bytes of the step-2 column -> slot map (0: binary search)
     int s3pk;       // element step 3 packs hash keys and positions (narrow tile-rows)
 };
+// symbolic = true runs steps 1-2 and the step-3 plan only (C's values and
+// in-tile columns stay unallocated); stats (when given) then holds C's exact
+// tile and nonzero counts and the numeric step's heavy-row scratch bytes.
+// On any error every temporary and every array of C allocated so far is
+// freed (C is left zeroed).
 int tsg_spgemm_planned(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, const tsg_tune *tune,
-                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out);
+                       tsg_dtiles *C, tsg_stats *stats, tsg_spgemm_plan *plan_out, bool symbolic = false);
+// tsg_spgemm_tile_rows with the symbolic option above.
+int tsg_spgemm_tile_rows_ex(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, int r0, int r1,
+                            const tsg_tune *tune, tsg_dtiles *C, tsg_stats *stats, bool symbolic);
 // Intermediate products of each A tile-row (device u64[A->tilem]).
 int tsg_tile_row_products(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, unsigned long long *d_out);
 // Device scratch bytes tsg_spgemm_enqueue needs for this C.
@@ -229,3 +237,6 @@ __device__ __forceinline__ unsigned tsg_xcd_block(unsigned bid, unsigned nblocks
 
 static inline unsigned tsg_div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 int tsg_validate_csr(tsg_ctx *ctx, const tsg_dcsr *a);
+// sorted = 0: row pointers and column range only (repeated or unsorted
+// columns allowed, as mmio_allinone leaves them).
+int tsg_validate_csr_mode(tsg_ctx *ctx, const tsg_dcsr *a, int sorted);

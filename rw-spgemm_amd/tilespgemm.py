@@ -129,6 +129,24 @@ class RequestInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Csum(C.Structure):
+    """tsg_csum: order-free checksums of a C tile matrix (include/tsg.h)."""
+    _fields_ = [("numtile", _ll), ("nnz", _ll), ("tiles", _ull), ("entries", _ull), ("entries_t", _ull)]
+
+    def key(self):
+        return (self.numtile, self.nnz, self.tiles, self.entries)
+
+    def add(self, o: "Csum"):
+        """Chunk sums add up (mod 2^64) to the whole product's."""
+        m = (1 << 64) - 1
+        self.numtile += o.numtile
+        self.nnz += o.nnz
+        self.tiles = (self.tiles + o.tiles) & m
+        self.entries = (self.entries + o.entries) & m
+        self.entries_t = (self.entries_t + o.entries_t) & m
+        return self
+
+
 class ServerStats(C.Structure):
     _fields_ = [("resident_bytes", _ll), ("reserved_bytes", _ll), ("peak_reserved_bytes", _ll), ("queued", _i),
                 ("fused_launches", _ll), ("single_launches", _ll), ("cached_bytes", _ll),
@@ -211,6 +229,11 @@ _sig("tsg_batch_autotune", _i, [_p, _i, C.POINTER(Tune), C.POINTER(_d)])
 _sig("tsg_batch_get_info", _i, [_p, C.POINTER(BatchInfo)])
 _sig("tsg_batch_result", _i, [_p, _i, C.POINTER(HTiles), C.POINTER(HCsr)])
 _sig("tsg_batch_destroy", None, [_p])
+_sig("tsg_batch_result_checksum", _i, [_p, _i, C.POINTER(Csum)])
+_sig("tsg_tiles_checksum", _i, [_p, C.POINTER(DTiles), _ll, _i, C.POINTER(Csum)])
+_sig("tsg_tiles_rows", _i, [_p, C.POINTER(DTiles), _i, C.POINTER(_i), _i, C.POINTER(HCsr)])
+_sig("tsg_server_result_checksum", _i, [_p, _ull, C.POINTER(Csum)])
+_sig("tsg_server_result_rows", _i, [_p, _ull, C.POINTER(_i), _i, C.POINTER(HCsr)])
 _sig("tsg_server_config_default", None, [C.POINTER(ServerConfig)])
 _sig("tsg_server_create", _i, [_i, C.POINTER(ServerConfig), C.POINTER(_p)])
 _sig("tsg_server_destroy", None, [_p])
@@ -223,6 +246,9 @@ _sig("tsg_server_release", _i, [_p, _ull])
 _sig("tsg_server_stats", _i, [_p, C.POINTER(ServerStats)])
 _sig("tsg_server_replay", _i, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), _i, C.POINTER(_ull),
                                C.POINTER(RequestInfo)])
+REPLAY_PROGRESS = C.CFUNCTYPE(None, _i, _i, _p)
+_sig("tsg_server_replay_progress", _i, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d), _i, C.POINTER(_ull),
+                                        C.POINTER(RequestInfo), REPLAY_PROGRESS, _p])
 _sig("tsg_gen_batch_spec", _i, [_i, _ull, _i, _i, C.POINTER(BatchSpec)])
 _sig("tsg_gen_batch_problem", _i, [C.POINTER(BatchSpec), C.POINTER(HCsr)])
 _sig("tsg_compat_set_device", _i, [_i])
@@ -398,6 +424,11 @@ class Batch:
         _check(lib.tsg_batch_result(self.h, i, None, C.byref(h)))
         return _take_hcsr(h)
 
+    def result_checksum(self, i) -> Csum:
+        cs = Csum()
+        _check(lib.tsg_batch_result_checksum(self.h, i, C.byref(cs)))
+        return cs
+
     def close(self):
         if self.h:
             lib.tsg_batch_destroy(self.h)
@@ -446,6 +477,18 @@ class Server:
         _check(lib.tsg_server_result_csr(self.h, rid, C.byref(h)))
         return _take_hcsr(h)
 
+    def result_checksum(self, rid) -> Csum:
+        cs = Csum()
+        _check(lib.tsg_server_result_checksum(self.h, rid, C.byref(cs)))
+        return cs
+
+    def result_rows(self, rid, rows) -> Csr:
+        """Rows `rows` of a finished request's C (a CSR with len(rows) rows)."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        h = HCsr()
+        _check(lib.tsg_server_result_rows(self.h, rid, r.ctypes.data_as(C.POINTER(_i)), len(r), C.byref(h)))
+        return _take_hcsr(h)
+
     def release(self, rid):
         _check(lib.tsg_server_release(self.h, rid))
 
@@ -454,14 +497,20 @@ class Server:
         _check(lib.tsg_server_stats(self.h, C.byref(st)))
         return st
 
-    def replay(self, handles, ops, arrival_ms, keep=False):
+    def replay(self, handles, ops, arrival_ms, keep=False, progress=None):
+        """Open-loop load (tsg_server_replay); progress(submitted, completed)
+        is called as requests are submitted and completed."""
         n = len(handles)
         hs = (_i * n)(*handles)
         os_ = (_i * n)(*ops)
         ar = (_d * n)(*arrival_ms)
         ids = (_ull * n)()
         infos = (RequestInfo * n)()
-        _check(lib.tsg_server_replay(self.h, n, hs, os_, ar, 1 if keep else 0, ids, infos))
+        if progress is None:
+            _check(lib.tsg_server_replay(self.h, n, hs, os_, ar, 1 if keep else 0, ids, infos))
+        else:
+            cb = REPLAY_PROGRESS(lambda sub, done, _u: progress(sub, done))
+            _check(lib.tsg_server_replay_progress(self.h, n, hs, os_, ar, 1 if keep else 0, ids, infos, cb, None))
         return [ids[i] for i in range(n)], [infos[i] for i in range(n)]
 
     def close(self):
@@ -824,6 +873,20 @@ class Context:
         _check(lib.tsg_spgemm_chunk_plan(self.h, C.byref(a), C.byref(b), max_products, max_chunks, bounds,
                                          C.byref(n), C.byref(tot)))
         return [bounds[i] for i in range(n.value + 1)], tot.value
+
+    def checksum(self, c: DTiles, row_base=0, jbase=0) -> Csum:
+        """Order-free checksums of C (tsg_tiles_checksum); a tile-row chunk's
+        row_base is 16 * its first tile-row."""
+        cs = Csum()
+        _check(lib.tsg_tiles_checksum(self.h, C.byref(c), row_base, jbase, C.byref(cs)))
+        return cs
+
+    def tile_rows_csr(self, c: DTiles, rows, jbase=0) -> Csr:
+        """Rows `rows` (local to c) of a C tile matrix as a CSR of len(rows) rows."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        h = HCsr()
+        _check(lib.tsg_tiles_rows(self.h, C.byref(c), jbase, r.ctypes.data_as(C.POINTER(_i)), len(r), C.byref(h)))
+        return _take_hcsr(h)
 
     def spgemm_tile_rows(self, a: DTiles, b: DTiles, r0: int, r1: int, tune: Tune | None = None, timed=True):
         c = DTiles()

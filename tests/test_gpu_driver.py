@@ -34,7 +34,7 @@ def write_mtx(path, csr, symmetric=False):
         f.write(f"%%MatrixMarket matrix coordinate real {'symmetric' if symmetric else 'general'}\n")
         f.write(f"{csr.m} {csr.n} {len(order)}\n")
         for k in order:
-            f.write(f"{r[k] + 1} {c[k] + 1} {v[k]!r}\n")
+            f.write(f"{int(r[k]) + 1} {int(c[k]) + 1} {float(v[k])!r}\n")
 
 
 def load_like_driver(spec, keep=False):
@@ -218,7 +218,8 @@ def test_plan_axis_sharing_and_estimates(kind, algo):
     st_b, info_b, _ = ts.reorder_plan(ts.PROBLEM_AA, algo, hb)
     if kind == ts.PROBLEM_AB and algo != "original":
         assert np.array_equal(col, info_b["row_perm"][0])
-        assert not np.array_equal(col, row)
+        if algo != "random":  # a seeded shuffle depends only on n and the seed
+            assert not np.array_equal(col, row)
     else:
         assert np.array_equal(col, row)
     assert info["input_tiles_A_before"] > 0 and info["estimated_c_tiles_before"] > 0

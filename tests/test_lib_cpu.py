@@ -22,6 +22,7 @@ def declared_functions():
         src = re.sub(r"//[^\n]*", "", src)
         src = re.sub(r"typedef\s+struct\s*\w*\s*\{.*?\}\s*\w+\s*;", "", src, flags=re.S)
         src = re.sub(r"enum\s*\{.*?\}\s*;", "", src, flags=re.S)
+        src = re.sub(r"typedef[^;]*\(\s*\*\s*\w+\s*\)[^;]*;", "", src)  # function-pointer types
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src, flags=re.M):
             names.add(m.group(1))
     return names

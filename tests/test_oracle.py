@@ -174,6 +174,22 @@ def test_oracle_rows_slice_matches_full():
     assert np.array_equal(v2, vc[rpc[30]:rpc[77]])
 
 
+def test_oracle_rowlist_matches_full():
+    """Sampled rows (any order, repeats) equal the same rows of the whole
+    product, bit for bit, values and structure-only."""
+    a = sprand(300, 260, 0.04, 22)
+    b = sprand(260, 310, 0.05, 23)
+    rpc, cc, vc = oracle.spgemm(a, b)
+    rows = np.array([299, 0, 17, 17, 150, 3, 298, 64], np.int32)
+    rp, c2, v2 = oracle.spgemm_rowlist(a, b, rows)
+    assert np.array_equal(np.diff(rp), np.diff(rpc)[rows])
+    for x, i in enumerate(rows):
+        assert np.array_equal(c2[rp[x]:rp[x + 1]], cc[rpc[i]:rpc[i + 1]])
+        assert np.array_equal(v2[rp[x]:rp[x + 1]].view(np.int64), vc[rpc[i]:rpc[i + 1]].view(np.int64))
+    rp3, c3, _ = oracle.spgemm_rowlist(a, b, rows, values=False)
+    assert np.array_equal(rp3, rp) and np.array_equal(c3, c2)
+
+
 # ------------------------------------------------------------------ exact fma order
 def _fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))  # correctly rounded

@@ -263,7 +263,7 @@ def write_matrix(path, kind, csr):
         f.write("%%MatrixMarket matrix coordinate real general\n")
         f.write(f"% order: {kind}\n{csr.m} {csr.n} {csr.nnz}\n")
         for r, c, v in zip(rows, csr.colidx, csr.val):
-            f.write(f"{r + 1} {c + 1} {v!r}\n")
+            f.write(f"{int(r) + 1} {int(c) + 1} {float(v)!r}\n")
 
 
 @needs_no_gpu
