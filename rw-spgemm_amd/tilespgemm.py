@@ -472,6 +472,18 @@ class Server:
         _check(lib.tsg_server_wait(self.h, rid, timeout_ms, C.byref(info)))
         return info
 
+    def poll(self, rid, timeout_ms) -> RequestInfo | None:
+        """The request's info once done, or None if it is not done within
+        timeout_ms (> 0)."""
+        info = RequestInfo()
+        info.handle = -1  # filled only when the request is done
+        st = lib.tsg_server_wait(self.h, rid, float(timeout_ms), C.byref(info))
+        if info.handle < 0:
+            if st == -6:  # TSG_ERR_STATE: not done yet
+                return None
+            _check(st)
+        return info
+
     def result_csr(self, rid) -> Csr:
         h = HCsr()
         _check(lib.tsg_server_result_csr(self.h, rid, C.byref(h)))
