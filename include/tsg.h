@@ -219,6 +219,20 @@ int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b,
                const tsg_tune *tune, tsg_dtiles *c, tsg_stats *stats);
 /* Step 1 alone: number of C tiles (structurally empty ones included). */
 int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, long long *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the C tile count of A*B exactly as
+ * tsg_spgemm_count_tiles computes it, enqueued on ctx's stream with no
+ * allocation, no memset and no host synchronization; the int64 count is
+ * written to d_out, device memory the caller owns.  Its scratch is a fixed
+ * TSG_NOALLOC_SCRATCH_BYTES device array per GPU, held until the process
+ * ends, so A may have at most TSG_NOALLOC_MAX_TILE_ROWS tile-rows (m up to
+ * about 67 million); larger inputs return TSG_ERR_ARG before any device
+ * work.  Every call on one GPU shares that array: calls that overlap on a
+ * GPU (several threads, or several streams) race. */
+#define TSG_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define TSG_NOALLOC_MAX_TILE_ROWS 4192256
+int tsg_spgemm_count_tiles_noalloc(tsg_ctx *ctx, const tsg_dtiles *a, const tsg_dtiles *b, long long *d_out);
 
 /* Products whose C exceeds the int32 offsets of the tile format (or the
  * memory at hand) run in chunks of A tile-rows.  The reference has no

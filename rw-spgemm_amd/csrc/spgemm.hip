@@ -25,6 +25,8 @@
 //   reference uses FP64 atomicAdd, src/tilespgemm-cuda.h:1670.)
 #include <string.h>
 
+#include <mutex>
+
 #include "tsg_internal.h"
 
 #define S1_WAVES 4
@@ -1769,6 +1771,103 @@ extern "C" int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *A, const t
     tsg_free(ctx, cnt);
     *out = tot;
     return st;
+}
+
+// ------------------------------------------------------------------ no-allocation tile count
+// tsg_spgemm_count_tiles_noalloc keeps its scratch in one fixed device
+// array per GPU (loaded with the code object, held until the process
+// ends): the per-tile-row C tile counts, then the per-block partial sums.
+// Host code never passes the array's name to a kernel (there it is only a
+// host-side stand-in); the device address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and kernels receive it
+// as an argument.
+#define NOALLOC_PARTS 1024  // partial sums (int64), at the end of the array
+static_assert(TSG_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch is a whole number of 256-byte lines");
+static_assert((long long)TSG_NOALLOC_MAX_TILE_ROWS * 4 + NOALLOC_PARTS * 8 <= TSG_NOALLOC_SCRATCH_BYTES,
+              "counts and partials fit the scratch array");
+static __device__ __align__(256) unsigned char scratch[TSG_NOALLOC_SCRATCH_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+
+static int noalloc_scratch(int device, unsigned char **out) {
+    *out = nullptr;
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        tsg_set_error("count_tiles_noalloc: device %d outside [0, %d)", device, NOALLOC_MAX_DEVICES);
+        return TSG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        int cur = -1;
+        TSG_HIP(hipGetDevice(&cur));
+        if (cur != device) TSG_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            tsg_set_error("count_tiles_noalloc: hipGetSymbolAddress(scratch) on device %d -> %s", device,
+                          hipGetErrorString(e));
+            return TSG_ERR_HIP;
+        }
+        g_scratch_dev[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return TSG_OK;
+}
+
+// Sum of n int32 counts: block b adds a grid-stride share into part[b]
+// (every slot written, so nothing needs clearing), then one block adds the
+// partials into *out.
+__global__ void __launch_bounds__(256)
+k_noalloc_partial(const int *__restrict__ cnt, int n, long long *__restrict__ part) {
+    long long s = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += cnt[i];
+    s = tsg_wave_sum(s);
+    __shared__ long long w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+__global__ void __launch_bounds__(256)
+k_noalloc_final(const long long *__restrict__ part, int nparts, long long *__restrict__ out) {
+    long long s = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+    s = tsg_wave_sum(s);
+    __shared__ long long w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = w[0] + w[1] + w[2] + w[3];
+}
+
+extern "C" int tsg_spgemm_count_tiles_noalloc(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B,
+                                              long long *d_out) {
+    if (!ctx || !A || !B || !d_out) { tsg_set_error("count_tiles_noalloc: null argument"); return TSG_ERR_ARG; }
+    if (A->n != B->m) { tsg_set_error("count_tiles_noalloc: inner dimensions differ"); return TSG_ERR_ARG; }
+    if (A->tilem < 0 || A->tilem > TSG_NOALLOC_MAX_TILE_ROWS) {
+        tsg_set_error("count_tiles_noalloc: A has %d tile-rows; the fixed scratch holds at most %d", A->tilem,
+                      TSG_NOALLOC_MAX_TILE_ROWS);
+        return TSG_ERR_ARG;
+    }
+    if (A->tilem > 0 && (!A->tile_ptr || !A->tile_colidx || !B->tile_ptr || !B->tile_colidx)) {
+        tsg_set_error("count_tiles_noalloc: missing tile index arrays");
+        return TSG_ERR_ARG;
+    }
+    unsigned char *scr;
+    TSG_TRY(noalloc_scratch(ctx->device, &scr));
+    int *cnt = reinterpret_cast<int *>(scr);
+    long long *part = reinterpret_cast<long long *>(scr + TSG_NOALLOC_SCRATCH_BYTES - NOALLOC_PARTS * 8);
+    int nparts = (int)std::min<long long>(NOALLOC_PARTS, std::max(1u, tsg_div_up(A->tilem, 256)));
+    if (A->tilem > 0) {
+        hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
+                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
+                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr);
+        TSG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_noalloc_partial, dim3(nparts), dim3(256), 0, ctx->stream, (const int *)cnt, A->tilem, part);
+    hipLaunchKernelGGL(k_noalloc_final, dim3(1), dim3(256), 0, ctx->stream, (const long long *)part, nparts, d_out);
+    TSG_LAUNCH_CHECK();
+    return TSG_OK;
 }
 
 static int ev_ms(hipEvent_t a, hipEvent_t b, double *ms) {

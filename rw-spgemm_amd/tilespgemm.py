@@ -201,6 +201,8 @@ _sig("tsg_h2d", _i, [_p, _p, _p, C.c_size_t])
 _sig("tsg_d2h", _i, [_p, _p, _p, C.c_size_t])
 _sig("tsg_copy_kernel", _i, [_p, _p, _p, C.c_size_t])
 _sig("tsg_spgemm_count_tiles", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), C.POINTER(_ll)])
+_sig("tsg_spgemm_count_tiles_noalloc", _i, [_p, C.POINTER(DTiles), C.POINTER(DTiles), _p])
+NOALLOC_MAX_TILE_ROWS = 4192256
 _sig("tsg_perm_validate", _i, [_p, _i, _p])
 _sig("tsg_perm_invert", _i, [_p, _i, _p, _p])
 _sig("tsg_order_degree", _i, [_p, C.POINTER(DCsr), _i, _p])
@@ -835,6 +837,11 @@ class Context:
         v = _ll(0)
         _check(lib.tsg_spgemm_count_tiles(self.h, C.byref(a), C.byref(b), C.byref(v)))
         return int(v.value)
+
+    def count_tiles_noalloc(self, a: DTiles, b: DTiles, d_out) -> None:
+        """Enqueues tsg_spgemm_count_tiles_noalloc: the count lands in the
+        caller's device int64 d_out (no allocation, no synchronization)."""
+        _check(lib.tsg_spgemm_count_tiles_noalloc(self.h, C.byref(a), C.byref(b), d_out))
 
     def order_degree(self, d: DCsr, symmetric_header=False) -> np.ndarray:
         p = self.dev_ints(n=d.m)
