@@ -346,6 +346,7 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 #define G2_UMAX 512    // C tiles of one segment held in LDS
 #define G2_PCHUNK 256  // A tiles per enumeration round
 #define G2_MAP_MAX 16384  // widest tile-row span with a direct column map (bytes of LDS)
+#define G2_MSTRIDE 9      // LDS words per C tile mask set / per thread B mask copy (8 used)
 
 // Most C tiles of a tile-row (capped at the step-2 segment size) and the
 // widest tile-column span of a tile-row: out[0], out[1] (atomic maxima).
@@ -365,13 +366,14 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
              const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
              int *__restrict__ Cnnz, int umax, int mapbytes) {
     // dynamic LDS, sized by the largest segment: the segment's C tile masks
-    // (two 16-bit rows per word, even row high), its tile columns, and when
-    // every segment has at most 256 tiles spanning at most mapbytes tile
+    // (two 16-bit rows per word, even row high; G2_MSTRIDE words per tile so
+    // that the lanes' tiles fall in distinct banks), its tile columns, and
+    // when every segment has at most 256 tiles spanning at most mapbytes tile
     // columns, a direct map tile column -> slot
     extern __shared__ __align__(16) unsigned char g2_smem[];
     unsigned *s_m = reinterpret_cast<unsigned *>(g2_smem);
-    int *s_ccol = reinterpret_cast<int *>(g2_smem + 32 * (size_t)umax);
-    unsigned char *s_map = g2_smem + 36 * (size_t)umax;
+    int *s_ccol = reinterpret_cast<int *>(g2_smem + 4 * G2_MSTRIDE * (size_t)umax);
+    unsigned char *s_map = g2_smem + (4 * G2_MSTRIDE + 4) * (size_t)umax;
     __shared__ int s_pre[G2_PCHUNK];
     __shared__ int s_b0[G2_PCHUNK];
     __shared__ int s_tot;
@@ -393,7 +395,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
         s_ccol[i] = J;
         if (mapbytes) s_map[J - jlo] = (unsigned char)i;
     }
-    for (int i = tid; i < u * 8; i += G2_THREADS) s_m[i] = 0u;
+    for (int i = tid; i < u * G2_MSTRIDE; i += G2_THREADS) s_m[i] = 0u;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
     unsigned short *bst = s_bst[tid];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
@@ -456,7 +458,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
                     am &= ~(1u << b);
                     m |= bst[15 - b];
                 }
-                atomicOr(&s_m[slot * 8 + (r >> 1)], m << ((r & 1) ? 0 : 16));
+                atomicOr(&s_m[slot * G2_MSTRIDE + (r >> 1)], m << ((r & 1) ? 0 : 16));
             }
         }
         __syncthreads();
@@ -465,7 +467,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     for (int s = tid; s < u; s += G2_THREADS) {
         unsigned w8[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) w8[k] = s_m[s * 8 + k];
+        for (int k = 0; k < 8; k++) w8[k] = s_m[s * G2_MSTRIDE + k];
         unsigned mk[8], pt[4];
         int sum = 0;
 #pragma unroll
@@ -1438,6 +1440,11 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             const unsigned rows_ = (unsigned)__ballot(am_ != 0) & 0xffffu;                                    \
             if (am_) s_rows[buf][__popc(rows_ & ((1u << lane) - 1))] = (unsigned char)lane;                   \
             if (lane == 0) s_nr[buf] = __popc(rows_);                                                         \
+            if (pan_ == 1) { /* one entry: its row and column for the direct path */                        \
+                const int r1_ = __ffs((int)rows_) - 1;                                                        \
+                const unsigned m1_ = (unsigned)__shfl((int)am_, r1_, 64);                                     \
+                if (lane == 0) s_rk[buf] = r1_ | ((15 - (31 - __clz((int)m1_))) << 8);                       \
+            }                                                                                                 \
         } else {                                                                                              \
             const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
             double *bv_ = s_bval + (buf) * bnnzs;                                                             \
@@ -1449,6 +1456,13 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
     } while (0)
 
 #define SROW(k, j) s_row[((k) - 1) * lamax + (j)]  // k = 1..7
+
+// ceil(2^32 / n) for n = 2..16: x / n == umulhi(x, g3_div_magic[n]) for every
+// x < 2^28 (a table, so each K-step reads one word instead of expanding a
+// 64-bit division)
+__constant__ unsigned g3_div_magic[17] = {0u,          0u,          0x80000000u, 0x55555556u, 0x40000000u, 0x33333334u,
+                                          0x2AAAAAABu, 0x24924925u, 0x20000000u, 0x1C71C71Du, 0x1999999Au, 0x1745D175u,
+                                          0x15555556u, 0x13B13B14u, 0x12492493u, 0x11111112u, 0x10000000u};
 
 __global__ void __launch_bounds__(G3_THREADS)
 k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
@@ -1477,6 +1491,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     __shared__ unsigned char s_ap[2][16];
     __shared__ unsigned char s_rows[2][16];
     __shared__ int s_nr[2];
+    __shared__ int s_rk[2];  // single-entry A tile of the stage: its row | column << 8
     __shared__ int s_npair;
     __shared__ int s_wne[G3_THREADS / 64];
     static_assert(G3_UMAX <= G3_THREADS, "one C tile per thread in the setup");
@@ -1582,10 +1597,39 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     // compute K-step j from buffer j&1
     auto compute = [&](int j) {
         const int buf = j & 1;
+        if (SROW(2, j) == 1) {
+            // A tile with one entry (rr, k): every task is row k of one B
+            // tile into row rr of its C tile, so one wave takes them with
+            // lane = B tile, no task compaction (the short dependency chain
+            // matters: the other waves wait at the K-step's barrier)
+            if (wave == 2 + (j & 1)) {
+                const int rk = s_rk[buf], rr = rk & 15, k = rk >> 8;
+                const int qi = lane;
+                if (qi < SROW(4, j)) {
+                    unsigned bm = s_bm[(buf * qb + qi) * 16 + k];
+                    if (bm) {
+                        const int slot = s_pslot[SROW(7, j) + qi];
+                        const unsigned cm = s_cmask[slot * 16 + rr];
+                        const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
+                        const double *brow = s_bval + buf * bnnzs + s_bnnz[buf * (qb + 1) + qi] +
+                                             s_bp[(buf * qb + qi) * 16 + k];
+                        const double a = s_aval[buf * amax];
+                        int jb = 0;
+                        while (bm) {  // B row k entries, ascending column
+                            const int bb = 31 - __clz(bm);
+                            bm &= ~(1u << bb);
+                            double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
+                            *acc = __builtin_fma(a, brow[jb++], *acc);
+                        }
+                    }
+                }
+            }
+            return;
+        }
         const int nr = s_nr[buf], nq = SROW(4, j), pbase = SROW(7, j);
         const int ncand = nq * nr;
         // x / nr by multiply-high (exact for x < 2^28, nr <= 16)
-        const unsigned magic = nr > 1 ? (unsigned)((0x100000000ULL + nr - 1) / (unsigned)nr) : 0u;
+        const unsigned magic = g3_div_magic[nr];
         const unsigned short *sbz = s_bnz + buf * qb;
         // candidate tasks (B tile qi, A row rr) meeting at least one nonempty
         // B row, compacted per wave into a register queue: position Q of the
@@ -1596,7 +1640,10 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         // qn.., inactive ones after them, so every lane receives exactly one
         // value and the permute needs no LDS memory.
         int qn = 0, qlo = 0, qhi = 0;
-        for (int c = wave * 64; c < ncand; c += G3_THREADS) {
+        // the first chunk goes to a different wave each K-step, so the
+        // single-chunk steps (one-row A tiles) do not all land on wave 0,
+        // which also stages the B metadata
+        for (int c = ((wave + j) & 3) * 64; c < ncand; c += G3_THREADS) {
             const int x = c + lane;
             int code = 0;
             bool act = false;
@@ -2031,7 +2078,7 @@ static int launch_step2(hipStream_t s, const TileOperands &op, const tsg_dtiles 
     }
     TSG_TRY(launch_segments<0>(s, C, w, -1, G2_UMAX, 0, w.seg2, P.s3_elem ? w.rowcls : nullptr));
     const int s2u = P.s2u > 0 ? P.s2u : G2_UMAX;
-    const size_t lds2 = 36 * (size_t)s2u + (((size_t)P.s2map + 15) & ~(size_t)15);
+    const size_t lds2 = (4 * G2_MSTRIDE + 4) * (size_t)s2u + (((size_t)P.s2map + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_step2_rows, dim3((unsigned)w.seg2_cap), dim3(G2_THREADS), lds2, s, op, (const int4 *)w.seg2,
                        (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
                        C->tile_nnz, s2u, P.s2map);
