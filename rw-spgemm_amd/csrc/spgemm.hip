@@ -361,6 +361,7 @@ k_tilerow_stats(const int *__restrict__ Cptr, const int *__restrict__ Ccol, int 
     }
 }
 
+static_assert(G2_PCHUNK == G2_THREADS, "one A tile of a chunk per thread");
 __global__ void __launch_bounds__(G2_THREADS)
 k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restrict__ nseg, const int *__restrict__ Cptr,
              const int *__restrict__ Ccol, unsigned short *__restrict__ Cmask, unsigned char *__restrict__ Cptr16,
@@ -398,6 +399,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     for (int i = tid; i < u * G2_MSTRIDE; i += G2_THREADS) s_m[i] = 0u;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
     unsigned short *bst = s_bst[tid];
+    __shared__ int s_wsum[G2_THREADS / 64];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
         const int np = min(G2_PCHUNK, a1 - pc);
         int len = 0, b0 = 0;
@@ -412,7 +414,6 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             len = b1 - b0;
         }
         // block-wide exclusive scan of len
-        __shared__ int s_wsum[G2_THREADS / 64];
         const int incl = tsg_wave_incl_scan(len);
         if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
         __syncthreads();
@@ -1492,7 +1493,6 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     __shared__ unsigned char s_rows[2][16];
     __shared__ int s_nr[2];
     __shared__ int s_rk[2];  // single-entry A tile of the stage: its row | column << 8
-    __shared__ int s_npair;
     __shared__ int s_wne[G3_THREADS / 64];
     static_assert(G3_UMAX <= G3_THREADS, "one C tile per thread in the setup");
     const int tid = threadIdx.x;
@@ -1532,7 +1532,6 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         }
         const int incl = tsg_wave_incl_scan(lb);
         if (lane < la) SROW(7, lane) = incl - lb;
-        if (lane == 63) s_npair = incl;
     }
     __syncthreads();
     // the ring's first three stages need only the row table: their loads
@@ -1572,12 +1571,16 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         }
     }
     __syncthreads();
-    for (int x = tid; x < s_npair; x += G3_THREADS) {
-        int j = 0;
-#pragma unroll
-        for (int st = G3_LAMAX / 2; st > 0; st >>= 1)
-            if (j + st < la && SROW(7, j + st) <= x) j += st;
-        const int J = op.Bcol[SROW(3, j) + (x - SROW(7, j))];
+    // pair (A tile j, B tile qi) at x = j * G3_QB + qi: no search for the A
+    // tile (B tile-rows hold at most G3_QB tiles); its slot index is the
+    // pair's rank SROW(7, j) + qi
+    static_assert(G3_QB == 32, "pair index split by shifts");
+#pragma unroll 2
+    for (int xx = tid; xx < la * G3_QB; xx += G3_THREADS) {
+        const int j = xx >> 5, qi = xx & 31;
+        if (qi >= SROW(4, j)) continue;
+        const int x = SROW(7, j) + qi;
+        const int J = op.Bcol[SROW(3, j) + qi];
         if (direct) {
             s_pslot[x] = s_map[J - jlo];
         } else {
