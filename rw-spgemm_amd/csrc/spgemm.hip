@@ -1321,18 +1321,23 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
 // B tile-rows fit in LDS (g3 class, decided by k_row_class).  The
 // tile-row's C values accumulate in LDS in their final order (tile by tile,
 // row by row, ascending column), so the write-out is one contiguous copy.
-// A tiles (I,K) are taken in ascending K; for each, the A tile and the
-// whole B tile-row K -- contiguous in row-major tile order -- are staged
-// into LDS with coalesced loads, and every (B tile q, nonempty A row r)
-// task multiplies A row r by the B tile into C tile slot(q), row r.  Tasks
-// of one K touch disjoint (tile, row) accumulators and a barrier separates
-// consecutive K, so every C entry receives its products in ascending k --
-// the order of the serial SPA -- with no atomics.
+// A tiles (I,K) are taken in ascending K; every (B tile q, nonempty A row
+// r) task multiplies A row r by the B tile into C tile slot(q), row r.
+// C row r belongs to wave r & 3: only that wave ever adds into it, and each
+// wave takes its tasks in ascending K, so every C entry receives its
+// products in ascending k -- the order of the serial SPA -- with no atomics
+// and no barrier between the K-steps for the accumulator's sake.
+//
+// An A tile with more than one entry has its A tile and its whole B
+// tile-row K -- contiguous in row-major tile order -- staged into LDS with
+// coalesced loads; an A tile with a single entry (r, k) needs row k of each
+// B tile only, which its row's wave loads into registers itself (the
+// stencil corner tiles: 18 of the 27 A tiles of a 27-point tile-row).
 //
 // The staging is software-pipelined: the tile-row's (K, B tile-row) table
-// is loaded once, stage j+1 sits in the second LDS buffer while stage j is
-// computed, and the global loads of stage j+2 are in flight meanwhile, so
-// each K costs one barrier and no exposed global latency.
+// is loaded once, a staged K-step sits in the second LDS buffer while the
+// previous one is computed, and the global loads of the step after next
+// are in flight meanwhile.  A barrier precedes each staged K-step only.
 #define G3_LAMAX 64  // A tiles per tile-row (row table built by wave 0)
 
 // Dynamic LDS layout of k_step3_rows, from the maxima over its rows.
@@ -1372,6 +1377,8 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 // loaded three K-steps before it is computed (register ring of 3).  The
 // roles share one set of 10 registers per stage: wave 0 keeps its two
 // 16-byte words as raw bits in d0:d1 and d2:d3 (moved, never computed on).
+// The ring runs over the staged (multi-entry) K-steps only: a stage is
+// loaded two staged steps before it is computed and stored one before.
 #define G3_SLOT_DECL(S)                                                   \
     double S##d0 = 0.0, S##d1 = 0.0, S##d2 = 0.0, S##d3 = 0.0;            \
     unsigned S##e0 = 0u, S##e1 = 0u
@@ -1433,18 +1440,9 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             if (lane + 64 < pan_) av_[lane + 64] = S##d1;                                                     \
             if (lane + 128 < pan_) av_[lane + 128] = S##d2;                                                   \
             if (lane + 192 < pan_) av_[lane + 192] = S##d3;                                                   \
-            const unsigned am_ = lane < 16 ? S##e0 & 0xffffu : 0u;                                            \
             if (lane < 16) {                                                                                  \
-                s_am[buf][lane] = (unsigned short)am_;                                                        \
+                s_am[buf][lane] = (unsigned short)S##e0;                                                      \
                 s_ap[buf][lane] = (unsigned char)S##e1;                                                       \
-            }                                                                                                 \
-            const unsigned rows_ = (unsigned)__ballot(am_ != 0) & 0xffffu;                                    \
-            if (am_) s_rows[buf][__popc(rows_ & ((1u << lane) - 1))] = (unsigned char)lane;                   \
-            if (lane == 0) s_nr[buf] = __popc(rows_);                                                         \
-            if (pan_ == 1) { /* one entry: its row and column for the direct path */                        \
-                const int r1_ = __ffs((int)rows_) - 1;                                                        \
-                const unsigned m1_ = (unsigned)__shfl((int)am_, r1_, 64);                                     \
-                if (lane == 0) s_rk[buf] = r1_ | ((15 - (31 - __clz((int)m1_))) << 8);                       \
             }                                                                                                 \
         } else {                                                                                              \
             const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
@@ -1456,14 +1454,75 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
         }                                                                                                     \
     } while (0)
 
-#define SROW(k, j) s_row[((k) - 1) * lamax + (j)]  // k = 1..7
+// Row table entry k of A tile j (k = 1..7): 1 its first value, 2 its entry
+// count, 3 the first tile and 4 the tile count of B tile-row K, 5 and 6 the
+// first value and value count of that B tile-row (for a single-entry tile:
+// 5 = its row | column << 8), 7 the rank of its first (A tile, B tile) pair.
+#define SROW(k, j) s_row[((k) - 1) * lamax + (j)]
 
-// ceil(2^32 / n) for n = 2..16: x / n == umulhi(x, g3_div_magic[n]) for every
-// x < 2^28 (a table, so each K-step reads one word instead of expanding a
-// 64-bit division)
-__constant__ unsigned g3_div_magic[17] = {0u,          0u,          0x80000000u, 0x55555556u, 0x40000000u, 0x33333334u,
-                                          0x2AAAAAABu, 0x24924925u, 0x20000000u, 0x1C71C71Du, 0x1999999Au, 0x1745D175u,
-                                          0x15555556u, 0x13B13B14u, 0x12492493u, 0x11111112u, 0x10000000u};
+// Single-entry K-steps of one wave, lane = B tile, in a two-level pipeline
+// over the steps the wave owns: the step after next has row k's mask, its
+// tile's value offset and row k's offset within the tile in flight (G3_L1:
+// sm_*); the next has row k's first three values and the A value (G3_L2:
+// sv_*).  Every address is in bounds (a lane past the B tile-row or with an
+// empty row k loads a valid element and computes nothing).
+#define G3_L1(j)                                                                      \
+    do {                                                                              \
+        const int rk_ = SROW(5, j), nq_ = SROW(4, j);                                 \
+        const int q_ = max(SROW(3, j) + min(lane, nq_ - 1), 0);                       \
+        const long long x_ = (long long)q_ * 16 + (rk_ >> 8);                         \
+        sm_m = op.Bmask[x_];                                                          \
+        sm_n = (unsigned)op.Bnnz[q_];                                                 \
+        sm_p = op.Bptr16[x_];                                                         \
+    } while (0)
+#define G3_L2(j)                                                                      \
+    do {                                                                              \
+        const int nb_ = __popc(sm_m), l_ = max(nb_ - 1, 0);                           \
+        const int pos_ = (int)(sm_n + sm_p), bs_ = nb_ > 0 ? pos_ : (int)sm_n;        \
+        sv_0 = op.Bval[bs_];                                                          \
+        sv_1 = op.Bval[bs_ + min(1, l_)];                                             \
+        sv_2 = op.Bval[bs_ + min(2, l_)];                                             \
+        sv_a = op.Aval[SROW(1, j)];                                                   \
+        sv_m = sm_m;                                                                  \
+        sv_p = pos_;                                                                  \
+    } while (0)
+// K-step j = (row rr, column k): row k of the lane's B tile into row rr of
+// the pair's C tile, ascending column (values past the third from memory);
+// then the pipeline moves one step on
+#define G3_SINGLE(j)                                                                  \
+    do {                                                                              \
+        const int rr_ = SROW(5, j) & 15;                                              \
+        unsigned bm_ = sv_m;                                                          \
+        if (lane < SROW(4, j) && bm_) {                                               \
+            const int slot_ = s_pslot[SROW(7, j) + lane];                             \
+            const unsigned cm_ = s_cmask[slot_ * 16 + rr_];                           \
+            double *ac_ = s_acc + s_coff[slot_] + s_cptr[slot_ * 16 + rr_];           \
+            int jb_ = 0;                                                              \
+            while (bm_) {                                                             \
+                const int bb_ = 31 - __clz(bm_);                                      \
+                bm_ &= ~(1u << bb_);                                                  \
+                const double b_ = jb_ == 0   ? sv_0                                   \
+                                  : jb_ == 1 ? sv_1                                   \
+                                  : jb_ == 2 ? sv_2                                   \
+                                             : op.Bval[sv_p + jb_];                   \
+                double *acc_ = ac_ + __popc(cm_ >> (bb_ + 1));                        \
+                *acc_ = __builtin_fma(sv_a, b_, *acc_);                               \
+                jb_++;                                                                \
+            }                                                                         \
+        }                                                                             \
+        sA = sB;                                                                      \
+        if (sA < 64) G3_L2(sA);                                                       \
+        sB = g3_pop(own);                                                             \
+        if (sB < 64) G3_L1(sB);                                                       \
+    } while (0)
+
+// Lowest set bit's index, cleared from m (64 when m is empty).
+__device__ __forceinline__ int g3_pop(unsigned long long &m) {
+    if (!m) return 64;
+    const int b = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    return b;
+}
 
 __global__ void __launch_bounds__(G3_THREADS)
 k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem,
@@ -1490,10 +1549,8 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     unsigned short *s_bnz = reinterpret_cast<unsigned short *>(g3_smem + off[12]);
     __shared__ unsigned short s_am[2][16];
     __shared__ unsigned char s_ap[2][16];
-    __shared__ unsigned char s_rows[2][16];
-    __shared__ int s_nr[2];
-    __shared__ int s_rk[2];  // single-entry A tile of the stage: its row | column << 8
     __shared__ int s_wne[G3_THREADS / 64];
+    __shared__ unsigned long long s_stg, s_own[G3_THREADS / 64];  // staged steps; singles per wave
     static_assert(G3_UMAX <= G3_THREADS, "one C tile per thread in the setup");
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1517,33 +1574,55 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     if (lane == 0) s_wne[wave] = __popcll(neb);
     // row table: one entry per A tile of the tile-row (wave 0, la <= 64)
     if (wave == 0) {
-        int lb = 0;
+        int lb = 0, pan = 0, rk = 0;
         if (lane < la) {
             const int p = a0 + lane, K = op.Acol[p];
             const int pa0 = op.Annz[p], q0 = op.Bptr[K], q1 = op.Bptr[K + 1];
-            const int pb0 = op.Bnnz[q0];
+            pan = op.Annz[p + 1] - pa0;
+            const int pb0 = op.Bnnz[q0], pb1 = op.Bnnz[q1];
+            // the row masks: a single entry's row and column
+            const uint4 *amk = reinterpret_cast<const uint4 *>(op.Amask + (long long)p * 16);
+            const uint4 m0 = amk[0], m1 = amk[1];
+            const unsigned w[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+#pragma unroll
+            for (int r = 15; r >= 0; r--) {
+                const unsigned m = (w[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+                if (m) rk = r | ((__clz((int)m) - 16) << 8);
+            }
             lb = q1 - q0;
             SROW(1, lane) = pa0;
-            SROW(2, lane) = op.Annz[p + 1] - pa0;
+            SROW(2, lane) = pan;
             SROW(3, lane) = q0;
             SROW(4, lane) = lb;
-            SROW(5, lane) = pb0;
-            SROW(6, lane) = op.Bnnz[q1] - pb0;
+            SROW(5, lane) = pan == 1 ? rk : pb0;
+            SROW(6, lane) = pb1 - pb0;
         }
         const int incl = tsg_wave_incl_scan(lb);
         if (lane < la) SROW(7, lane) = incl - lb;
+        const unsigned long long stg = __ballot(pan > 1);
+        unsigned long long ow[G3_THREADS / 64];
+#pragma unroll
+        for (int w = 0; w < G3_THREADS / 64; w++) ow[w] = __ballot(pan == 1 && (rk & 3) == w);
+        if (lane == 0) {
+            s_stg = stg;
+#pragma unroll
+            for (int w = 0; w < G3_THREADS / 64; w++) s_own[w] = ow[w];
+        }
     }
     __syncthreads();
     // the ring's first three stages need only the row table: their loads
     // are in flight while the C tile slots and pair slots are set up
+    unsigned long long rest = s_stg, own = s_own[wave];
+    int fa = g3_pop(rest), fb = g3_pop(rest);  // staged steps 0 and 1 (64: none)
     G3_SLOT_DECL(R0_);
     G3_SLOT_DECL(R1_);
-    G3_SLOT_DECL(R2_);
-    if (la > 0) {
-        G3_LOAD(0, R0_);
-        if (la > 1) G3_LOAD(1, R1_);
-        if (la > 2) G3_LOAD(2, R2_);
-    }
+    if (fa < 64) G3_LOAD(fa, R0_);
+    if (fb < 64) G3_LOAD(fb, R1_);
+    unsigned sm_m = 0u, sm_n = 0u, sm_p = 0u, sv_m = 0u;
+    int sv_p = 0;
+    double sv_0 = 0.0, sv_1 = 0.0, sv_2 = 0.0, sv_a = 0.0;
+    int sA = g3_pop(own), sB = g3_pop(own);  // this wave's next two single-entry steps
+    if (sA < 64) G3_L1(sA);
     int ci = __popcll(neb & ((1ull << lane) - 1ull));
     for (int w = 0; w < wave; w++) ci += s_wne[w];
     int un = 0;
@@ -1594,65 +1673,31 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     }
     __syncthreads();  // the map area becomes the accumulator
     for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
-    if (la > 0) G3_STORE(0, R0_, 0);
+    if (sA < 64) G3_L2(sA);
+    if (sB < 64) G3_L1(sB);
+    if (fa < 64) G3_STORE(fa, R0_, 0);
     __syncthreads();
 
-    // compute K-step j from buffer j&1
-    auto compute = [&](int j) {
-        const int buf = j & 1;
-        if (SROW(2, j) == 1) {
-            // A tile with one entry (rr, k): every task is row k of one B
-            // tile into row rr of its C tile, so one wave takes them with
-            // lane = B tile, no task compaction (the short dependency chain
-            // matters: the other waves wait at the K-step's barrier)
-            if (wave == 2 + (j & 1)) {
-                const int rk = s_rk[buf], rr = rk & 15, k = rk >> 8;
-                const int qi = lane;
-                if (qi < SROW(4, j)) {
-                    unsigned bm = s_bm[(buf * qb + qi) * 16 + k];
-                    if (bm) {
-                        const int slot = s_pslot[SROW(7, j) + qi];
-                        const unsigned cm = s_cmask[slot * 16 + rr];
-                        const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
-                        const double *brow = s_bval + buf * bnnzs + s_bnnz[buf * (qb + 1) + qi] +
-                                             s_bp[(buf * qb + qi) * 16 + k];
-                        const double a = s_aval[buf * amax];
-                        int jb = 0;
-                        while (bm) {  // B row k entries, ascending column
-                            const int bb = 31 - __clz(bm);
-                            bm &= ~(1u << bb);
-                            double *acc = &s_acc[cbase + __popc(cm >> (bb + 1))];
-                            *acc = __builtin_fma(a, brow[jb++], *acc);
-                        }
-                    }
-                }
-            }
-            return;
-        }
-        const int nr = s_nr[buf], nq = SROW(4, j), pbase = SROW(7, j);
-        const int ncand = nq * nr;
-        // x / nr by multiply-high (exact for x < 2^28, nr <= 16)
-        const unsigned magic = g3_div_magic[nr];
+    // compute staged K-step j from buffer buf: the tasks of this wave's rows
+    auto compute = [&](int j, int buf) {
+        const int nq = SROW(4, j), pbase = SROW(7, j);
+        const int ncand = nq * 4;
         const unsigned short *sbz = s_bnz + buf * qb;
-        // candidate tasks (B tile qi, A row rr) meeting at least one nonempty
-        // B row, compacted per wave into a register queue: position Q of the
-        // queue is lane Q & 63 of qlo (Q < 64) or qhi (Q >= 64).  A wave has
-        // at most two chunks of 64 candidates (ncand <= G3_QB * 16 = 512), so
+        // candidate tasks (B tile qi, A row wave + 4i) meeting at least one
+        // nonempty B row, compacted into a register queue: position Q of the
+        // queue is lane Q & 63 of qlo (Q < 64) or qhi (Q >= 64).  There are
+        // at most two chunks of 64 candidates (ncand <= G3_QB * 4 = 128), so
         // qn <= 64 before each chunk.  Each chunk is moved by one forward
         // permute over all 64 lanes: active lanes to the queue positions
         // qn.., inactive ones after them, so every lane receives exactly one
         // value and the permute needs no LDS memory.
         int qn = 0, qlo = 0, qhi = 0;
-        // the first chunk goes to a different wave each K-step, so the
-        // single-chunk steps (one-row A tiles) do not all land on wave 0,
-        // which also stages the B metadata
-        for (int c = ((wave + j) & 3) * 64; c < ncand; c += G3_THREADS) {
+        for (int c = 0; c < ncand; c += 64) {
             const int x = c + lane;
             int code = 0;
             bool act = false;
             if (x < ncand) {
-                const int qi = nr > 1 ? (int)__umulhi((unsigned)x, magic) : x;
-                const int rr = s_rows[buf][x - qi * nr];
+                const int qi = x >> 2, rr = wave + 4 * (x & 3);
                 act = (s_am[buf][rr] & sbz[qi]) != 0;
                 code = (qi << 4) | rr;
             }
@@ -1699,19 +1744,32 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
             }
         }
     };
-#define G3_ITER(j, SN, SR)                                           \
-    if ((j) < la) {                                                  \
-        compute(j);                                                  \
-        if ((j) + 1 < la) G3_STORE((j) + 1, SN, ((j) + 1) & 1);      \
-        if ((j) + 3 < la) G3_LOAD((j) + 3, SR);                      \
-        __syncthreads();                                             \
+    // single-entry steps before the first staged step
+    while (sA < fa) G3_SINGLE(sA);
+    // staged step fa (number f) from buffer f & 1 and ring slot SR; then the
+    // next staged step fb is stored from slot SN, the singles between them
+    // are taken, and staged step fc is loaded into slot SR.  A barrier
+    // precedes every staged step: its stores are visible, and the buffer it
+    // overwrote was last read two staged steps before.
+    int f = 0;
+#define G3_ITER(SN, SR)                                 \
+    if (fa < 64) {                                      \
+        compute(fa, f & 1);                             \
+        if (fb < 64) G3_STORE(fb, SN, (f + 1) & 1);     \
+        const int fc = g3_pop(rest);                    \
+        while (sA < fb) G3_SINGLE(sA);                  \
+        if (fc < 64) G3_LOAD(fc, SR);                   \
+        if (fb < 64) __syncthreads();                   \
+        fa = fb;                                        \
+        fb = fc;                                        \
+        f++;                                            \
     }
-    for (int j0 = 0; j0 < la; j0 += 3) {
-        G3_ITER(j0, R1_, R0_)
-        G3_ITER(j0 + 1, R2_, R1_)
-        G3_ITER(j0 + 2, R0_, R2_)
+    while (fa < 64) {
+        G3_ITER(R1_, R0_)
+        G3_ITER(R0_, R1_)
     }
 #undef G3_ITER
+    __syncthreads();  // every wave's rows are complete
     // write out: values are already in final order; columns from the masks
     for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
     for (int x = tid; x < un * 16; x += G3_THREADS) {
