@@ -341,7 +341,9 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 // is visited exactly once, flattened over the threads; no intersection
 // search is needed.  Row r of the pair's product pattern, OR over k in
 // Amask[r] of Bmask[k], is OR-ed into C tile J's row mask held in LDS (OR
-// is order-free, so LDS atomics are exact).
+// is order-free, so LDS atomics are exact).  A tiles with a single entry
+// (r, k) contribute B row k to C row r alone; their pairs are enumerated
+// before the others' so that a wave's lanes take one kind of pair.
 #define G2_THREADS 256
 #define G2_UMAX 512    // C tiles of one segment held in LDS
 #define G2_PCHUNK 256  // A tiles per enumeration round
@@ -375,8 +377,14 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     unsigned *s_m = reinterpret_cast<unsigned *>(g2_smem);
     int *s_ccol = reinterpret_cast<int *>(g2_smem + 4 * G2_MSTRIDE * (size_t)umax);
     unsigned char *s_map = g2_smem + (4 * G2_MSTRIDE + 4) * (size_t)umax;
+    // per enumeration position (single-entry A tiles first): first pair,
+    // first B tile, A tile (offset in the chunk), single entry (row |
+    // column << 4, or 255)
     __shared__ int s_pre[G2_PCHUNK];
     __shared__ int s_b0[G2_PCHUNK];
+    __shared__ unsigned char s_pid[G2_PCHUNK];
+    __shared__ unsigned char s_rk[G2_PCHUNK];
+    __shared__ int s_wsg[G2_THREADS / 64];
     __shared__ int s_tot;
     __shared__ unsigned short s_bst[G2_THREADS][16];
     const int tid = threadIdx.x;
@@ -402,8 +410,21 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     __shared__ int s_wsum[G2_THREADS / 64];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
         const int np = min(G2_PCHUNK, a1 - pc);
-        int len = 0, b0 = 0;
+        int len = 0, b0 = 0, rk = 255;
         if (tid < np) {
+            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)(pc + tid) * 16);
+            const uint4 aA = am4[0], aB = am4[1];
+            const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
+            int nz = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                nz += __popc(aw[i]);
+                if (aw[i]) {
+                    const int bit = 31 - __clz(aw[i]);  // row 2i in the low half, 2i+1 in the high
+                    rk = (2 * i + (bit >> 4)) | ((15 - (bit & 15)) << 4);
+                }
+            }
+            if (nz != 1) rk = 255;
             const int K = op.Acol[pc + tid];
             b0 = op.Bptr[K];
             int b1 = op.Bptr[K + 1];
@@ -413,14 +434,41 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             }
             len = b1 - b0;
         }
-        // block-wide exclusive scan of len
+        // enumeration positions: single-entry A tiles first, in thread order
+        {
+            const bool sgl = tid < np && rk != 255, dns = tid < np && rk == 255;
+            const unsigned long long sb = __ballot(sgl), db = __ballot(dns);
+            const unsigned long long lt = (1ull << (tid & 63)) - 1ull;
+            if ((tid & 63) == 0) {
+                s_wsg[tid >> 6] = __popcll(sb);
+                s_wsum[tid >> 6] = __popcll(db);
+            }
+            __syncthreads();
+            int ns = 0, sbef = 0, dbef = 0;
+            for (int w2 = 0; w2 < G2_THREADS / 64; w2++) {
+                ns += s_wsg[w2];
+                if (w2 < (tid >> 6)) {
+                    sbef += s_wsg[w2];
+                    dbef += s_wsum[w2];
+                }
+            }
+            const int pos = sgl ? sbef + __popcll(sb & lt) : ns + dbef + __popcll(db & lt);
+            if (tid < np) {
+                s_pre[pos] = len;
+                s_b0[pos] = b0;
+                s_pid[pos] = (unsigned char)tid;
+                s_rk[pos] = (unsigned char)rk;
+            }
+            __syncthreads();
+            len = tid < np ? s_pre[tid] : 0;
+        }
+        // block-wide exclusive scan of len in position order
         const int incl = tsg_wave_incl_scan(len);
         if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
         __syncthreads();
         int woff = 0;
         for (int w2 = 0; w2 < (tid >> 6); w2++) woff += s_wsum[w2];
         s_pre[tid] = woff + incl - len;
-        s_b0[tid] = b0;
         if (tid == G2_THREADS - 1) s_tot = woff + incl;
         __syncthreads();
         const int total = s_tot;
@@ -442,11 +490,18 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
                 }
                 slot = lo;
             }
+            const int rk1 = s_rk[p];
+            if (rk1 != 255) {  // single entry (r, k): B row k into C row r
+                const int r = rk1 & 15;
+                const unsigned m = op.Bmask[(long long)q * 16 + (rk1 >> 4)];
+                atomicOr(&s_m[slot * G2_MSTRIDE + (r >> 1)], m << ((r & 1) ? 0 : 16));
+                continue;
+            }
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q * 16);
             const uint4 bA = bm4[0], bB = bm4[1];
             *reinterpret_cast<uint4 *>(bst) = bA;
             *reinterpret_cast<uint4 *>(bst + 8) = bB;
-            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)(pc + p) * 16);
+            const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)(pc + s_pid[p]) * 16);
             const uint4 aA = am4[0], aB = am4[1];
             const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
 #pragma unroll
