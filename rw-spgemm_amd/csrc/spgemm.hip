@@ -68,6 +68,43 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     jmin = tsg_wave_min(jmin);
     jmax = tsg_wave_max(jmax);
 
+    // a row spanning 2..64 windows (a periodic grid's boundary rows) first
+    // finds the windows its candidates hit and visits only those
+    constexpr int WSH = __builtin_ctz(32 * BMW);  // log2 of the columns per window
+    unsigned long long wmask = ~0ull;
+    if (jmax >= jmin && ((jmax - jmin) >> WSH) >= 1 && ((jmax - jmin) >> WSH) < 64) {
+        unsigned wlo = 0u, whi = 0u;
+        for (int c0 = a0; c0 < a1; c0 += 64) {
+            const int p = c0 + lane;
+            int b0 = 0, len = 0;
+            if (p < a1) {
+                const int K = Acol[p];
+                b0 = Bptr[K];
+                len = Bptr[K + 1] - b0;
+            }
+            const int incl = tsg_wave_incl_scan(len);
+            const int tot = __shfl(incl, 63, 64);
+            pre[lane] = incl - len;
+            pb0[lane] = b0;
+            tsg_wave_sync();
+            for (int c = lane; c < tot; c += 64) {
+                int lo = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (pre[lo + st] <= c) lo += st;
+                const int wi = (Bcol[pb0[lo] + (c - pre[lo])] - jmin) >> WSH;
+                if (wi < 32) wlo |= 1u << wi; else whi |= 1u << (wi - 32);
+            }
+            tsg_wave_sync();
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            wlo |= __shfl_xor(wlo, o, 64);
+            whi |= __shfl_xor(whi, o, 64);
+        }
+        wmask = ((unsigned long long)whi << 32) | wlo;
+    }
+
     int total = 0;
     const int base = FILL ? Cptr_or_cnt[I] : 0;
     // a row whose candidates fit one window keeps the count pass's bitmap
@@ -75,6 +112,8 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     const bool kept = bmstore != nullptr && jmax >= jmin && jmax - jmin < 32 * BMW;
     unsigned *keep = kept ? bmstore + (long long)I * BMW : nullptr;
     for (int w0 = jmin; w0 <= jmax; w0 += 32 * BMW) {
+        const int wi = (w0 - jmin) >> WSH;
+        if (wi < 64 && !((wmask >> wi) & 1ull)) continue;  // no candidate in this window
         const int nw = min(BMW, ((jmax - w0) >> 5) + 1);
         if (FILL && kept) {
             for (int k = lane; k < nw; k += 64) bm[k] = keep[k];
@@ -350,16 +389,19 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 #define G2_MAP_MAX 16384  // widest tile-row span with a direct column map (bytes of LDS)
 #define G2_MSTRIDE 9      // LDS words per C tile mask set / per thread B mask copy (8 used)
 
-// Most C tiles of a tile-row (capped at the step-2 segment size) and the
-// widest tile-column span of a tile-row: out[0], out[1] (atomic maxima).
+// Most C tiles of a tile-row (capped at the step-2 segment size), the
+// widest tile-column span of a tile-row, and the widest span of at most
+// G2_MAP_MAX: out[0], out[1], out[2] (atomic maxima).
 __global__ void __launch_bounds__(256)
 k_tilerow_stats(const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem, int *__restrict__ out) {
     const int I = blockIdx.x * 256 + threadIdx.x;
     if (I >= tilem) return;
     const int c0 = Cptr[I], c1 = Cptr[I + 1];
     if (c1 > c0) {
+        const int span = Ccol[c1 - 1] - Ccol[c0] + 1;
         atomicMax(&out[0], min(c1 - c0, G2_UMAX));
-        atomicMax(&out[1], Ccol[c1 - 1] - Ccol[c0] + 1);
+        atomicMax(&out[1], span);
+        if (span <= G2_MAP_MAX) atomicMax(&out[2], span);
     }
 }
 
@@ -371,8 +413,9 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     // dynamic LDS, sized by the largest segment: the segment's C tile masks
     // (two 16-bit rows per word, even row high; G2_MSTRIDE words per tile so
     // that the lanes' tiles fall in distinct banks), its tile columns, and
-    // when every segment has at most 256 tiles spanning at most mapbytes tile
-    // columns, a direct map tile column -> slot
+    // when every segment has at most 256 tiles, mapbytes bytes for a direct
+    // map tile column -> slot, used by the segments spanning at most
+    // mapbytes tile columns (the others search the tile columns)
     extern __shared__ __align__(16) unsigned char g2_smem[];
     unsigned *s_m = reinterpret_cast<unsigned *>(g2_smem);
     int *s_ccol = reinterpret_cast<int *>(g2_smem + 4 * G2_MSTRIDE * (size_t)umax);
@@ -399,10 +442,11 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     const int c0 = Cptr[I] + sg.y, u = sg.z - sg.y;
     const bool whole = sg.y == 0 && sg.z == Cptr[I + 1] - Cptr[I];
     const int jlo = Ccol[c0], jhi = Ccol[c0 + u - 1];
+    const bool usemap = jhi - jlo < mapbytes;
     for (int i = tid; i < u; i += G2_THREADS) {
         const int J = Ccol[c0 + i];
         s_ccol[i] = J;
-        if (mapbytes) s_map[J - jlo] = (unsigned char)i;
+        if (usemap) s_map[J - jlo] = (unsigned char)i;
     }
     for (int i = tid; i < u * G2_MSTRIDE; i += G2_THREADS) s_m[i] = 0u;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
@@ -480,7 +524,7 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             const int q = s_b0[p] + (x - s_pre[p]);
             const int J = op.Bcol[q];  // J is in the segment's step-1 list
             int slot;
-            if (mapbytes) {
+            if (usemap) {
                 slot = s_map[J - jlo];
             } else {
                 int lo = 0, hi = u - 1;
@@ -2362,11 +2406,11 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
         hipLaunchKernelGGL(k_tilerow_stats, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, C->tile_ptr,
                            C->tile_colidx, C->tilem, st2);
         TSG_LAUNCH_CHECK();
-        long long v = 0;
-        TSG_TRY(tsg_read_scalar_i64(ctx, st2, 1, &v));
-        const int umax = (int)(v & 0xffffffffLL), span = (int)(v >> 32);
+        long long v[2] = {0, 0};
+        TSG_TRY(tsg_read_i64_pair(ctx, st2, v));
+        const int umax = (int)(v[0] & 0xffffffffLL), span = (int)(v[0] >> 32), mspan = (int)(v[1] & 0xffffffffLL);
         P.s2u = std::max(1, umax);
-        P.s2map = (umax <= 256 && span <= G2_MAP_MAX) ? span : 0;
+        P.s2map = umax <= 256 ? mspan : 0;
         P.s3pk = span < G3E_PK_SPAN ? 1 : 0;
     }
     // ---- step 2
