@@ -1562,9 +1562,10 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 // Single-entry K-steps of one wave, lane = B tile, in a two-level pipeline
 // over the steps the wave owns: the step after next has row k's mask, its
 // tile's value offset and row k's offset within the tile in flight (G3_L1:
-// sm_*); the next has row k's first three values and the A value (G3_L2:
-// sv_*).  Every address is in bounds (a lane past the B tile-row or with an
-// empty row k loads a valid element and computes nothing).
+// sm_*); the next has row k's first two values, the A value, and its C
+// row's mask and accumulator offset from the pair's slot (G3_L2: sv_*).
+// Every address is in bounds (a lane past the B tile-row or with an empty
+// row k loads a valid element and computes nothing).
 #define G3_L1(j)                                                                      \
     do {                                                                              \
         const int rk_ = SROW(5, j), nq_ = SROW(4, j);                                 \
@@ -1576,35 +1577,31 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
     } while (0)
 #define G3_L2(j)                                                                      \
     do {                                                                              \
-        const int nb_ = __popc(sm_m), l_ = max(nb_ - 1, 0);                           \
+        const int nb_ = __popc(sm_m), nq_ = SROW(4, j), rr_ = SROW(5, j) & 15;        \
         const int pos_ = (int)(sm_n + sm_p), bs_ = nb_ > 0 ? pos_ : (int)sm_n;        \
         sv_0 = op.Bval[bs_];                                                          \
-        sv_1 = op.Bval[bs_ + min(1, l_)];                                             \
-        sv_2 = op.Bval[bs_ + min(2, l_)];                                             \
+        sv_1 = op.Bval[bs_ + min(1, max(nb_ - 1, 0))];                                \
         sv_a = op.Aval[SROW(1, j)];                                                   \
         sv_m = sm_m;                                                                  \
         sv_p = pos_;                                                                  \
+        const int sl_ = nq_ > 0 ? s_pslot[SROW(7, j) + min(lane, nq_ - 1)] : 0;       \
+        sv_cm = s_cmask[sl_ * 16 + rr_];                                              \
+        sv_cb = s_coff[sl_] + s_cptr[sl_ * 16 + rr_];                                 \
     } while (0)
 // K-step j = (row rr, column k): row k of the lane's B tile into row rr of
-// the pair's C tile, ascending column (values past the third from memory);
+// the pair's C tile, ascending column (values past the second from memory);
 // then the pipeline moves one step on
 #define G3_SINGLE(j)                                                                  \
     do {                                                                              \
-        const int rr_ = SROW(5, j) & 15;                                              \
         unsigned bm_ = sv_m;                                                          \
         if (lane < SROW(4, j) && bm_) {                                               \
-            const int slot_ = s_pslot[SROW(7, j) + lane];                             \
-            const unsigned cm_ = s_cmask[slot_ * 16 + rr_];                           \
-            double *ac_ = s_acc + s_coff[slot_] + s_cptr[slot_ * 16 + rr_];           \
+            double *ac_ = s_acc + sv_cb;                                              \
             int jb_ = 0;                                                              \
             while (bm_) {                                                             \
                 const int bb_ = 31 - __clz(bm_);                                      \
                 bm_ &= ~(1u << bb_);                                                  \
-                const double b_ = jb_ == 0   ? sv_0                                   \
-                                  : jb_ == 1 ? sv_1                                   \
-                                  : jb_ == 2 ? sv_2                                   \
-                                             : op.Bval[sv_p + jb_];                   \
-                double *acc_ = ac_ + __popc(cm_ >> (bb_ + 1));                        \
+                const double b_ = jb_ == 0 ? sv_0 : jb_ == 1 ? sv_1 : op.Bval[sv_p + jb_]; \
+                double *acc_ = ac_ + __popc(sv_cm >> (bb_ + 1));                      \
                 *acc_ = __builtin_fma(sv_a, b_, *acc_);                               \
                 jb_++;                                                                \
             }                                                                         \
@@ -1717,9 +1714,9 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     G3_SLOT_DECL(R1_);
     if (fa < 64) G3_LOAD(fa, R0_);
     if (fb < 64) G3_LOAD(fb, R1_);
-    unsigned sm_m = 0u, sm_n = 0u, sm_p = 0u, sv_m = 0u;
-    int sv_p = 0;
-    double sv_0 = 0.0, sv_1 = 0.0, sv_2 = 0.0, sv_a = 0.0;
+    unsigned sm_m = 0u, sm_n = 0u, sm_p = 0u, sv_m = 0u, sv_cm = 0u;
+    int sv_p = 0, sv_cb = 0;
+    double sv_0 = 0.0, sv_1 = 0.0, sv_a = 0.0;
     int sA = g3_pop(own), sB = g3_pop(own);  // this wave's next two single-entry steps
     if (sA < 64) G3_L1(sA);
     int ci = __popcll(neb & ((1ull << lane) - 1ull));
