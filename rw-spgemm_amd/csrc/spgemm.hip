@@ -1422,8 +1422,8 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
 // row by row, ascending column), so the write-out is one contiguous copy.
 // A tiles (I,K) are taken in ascending K; every (B tile q, nonempty A row
 // r) task multiplies A row r by the B tile into C tile slot(q), row r.
-// C row r belongs to wave r & 3: only that wave ever adds into it, and each
-// wave takes its tasks in ascending K, so every C entry receives its
+// C row r belongs to wave g3_owner(r): only that wave ever adds into it,
+// and each wave takes its tasks in ascending K, so every C entry receives its
 // products in ascending k -- the order of the serial SPA -- with no atomics
 // and no barrier between the K-steps for the accumulator's sake.
 //
@@ -1558,6 +1558,12 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 // first value and value count of that B tile-row (for a single-entry tile:
 // 5 = its row | column << 8), 7 the rank of its first (A tile, B tile) pair.
 #define SROW(k, j) s_row[((k) - 1) * lamax + (j)]
+
+// Wave owning C row r: rows r = 4i + c belong to wave (2, 0, 1, 3)[c], so the
+// rows whose single-entry tiles a stencil produces (0 and 15) fall on the
+// waves that stage only B values; g3_rbase(w) = the c of wave w.
+__device__ __forceinline__ int g3_owner(int r) { return (0xD2 >> (2 * (r & 3))) & 3; }
+__device__ __forceinline__ int g3_rbase(int w) { return (0xC9 >> (2 * w)) & 3; }
 
 // Single-entry K-steps of one wave, lane = B tile, in a two-level pipeline
 // over the steps the wave owns: the step after next has row k's mask, its
@@ -1698,7 +1704,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         const unsigned long long stg = __ballot(pan > 1);
         unsigned long long ow[G3_THREADS / 64];
 #pragma unroll
-        for (int w = 0; w < G3_THREADS / 64; w++) ow[w] = __ballot(pan == 1 && (rk & 3) == w);
+        for (int w = 0; w < G3_THREADS / 64; w++) ow[w] = __ballot(pan == 1 && g3_owner(rk & 15) == w);
         if (lane == 0) {
             s_stg = stg;
 #pragma unroll
@@ -1779,7 +1785,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         const int nq = SROW(4, j), pbase = SROW(7, j);
         const int ncand = nq * 4;
         const unsigned short *sbz = s_bnz + buf * qb;
-        // candidate tasks (B tile qi, A row wave + 4i) meeting at least one
+        // candidate tasks (B tile qi, A row rb + 4i) meeting at least one
         // nonempty B row, compacted into a register queue: position Q of the
         // queue is lane Q & 63 of qlo (Q < 64) or qhi (Q >= 64).  There are
         // at most two chunks of 64 candidates (ncand <= G3_QB * 4 = 128), so
@@ -1788,12 +1794,13 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         // qn.., inactive ones after them, so every lane receives exactly one
         // value and the permute needs no LDS memory.
         int qn = 0, qlo = 0, qhi = 0;
+        const int rb = g3_rbase(wave);
         for (int c = 0; c < ncand; c += 64) {
             const int x = c + lane;
             int code = 0;
             bool act = false;
             if (x < ncand) {
-                const int qi = x >> 2, rr = wave + 4 * (x & 3);
+                const int qi = x >> 2, rr = rb + 4 * (x & 3);
                 act = (s_am[buf][rr] & sbz[qi]) != 0;
                 code = (qi << 4) | rr;
             }
