@@ -1665,12 +1665,18 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     // nonempty ones get accumulator metadata (offset, masks, row offsets),
     // at their rank among the nonempty tiles: a task is active only when
     // one of its products exists, so no task lands in an empty C tile.
+    // (their masks and row offsets are loaded here, stored after the barrier)
     int ccol = 0, cnz0 = 0;
     bool ne = false;
+    uint4 cmk0 = make_uint4(0u, 0u, 0u, 0u), cmk1 = cmk0, cpt = cmk0;
     if (tid < u) {
         ccol = Ccol[c0 + tid];
         cnz0 = Cnnz[c0 + tid];
         ne = Cnnz[c0 + tid + 1] > cnz0;
+        const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + tid) * 16);
+        cmk0 = cm[0];
+        cmk1 = cm[1];
+        cpt = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + tid) * 16)[0];
     }
     const unsigned long long neb = __ballot(ne);
     if (lane == 0) s_wne[wave] = __popcll(neb);
@@ -1745,10 +1751,9 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (direct) s_map[ccol - jlo] = (unsigned char)ci;
         if (ne) {
             s_coff[ci] = (unsigned short)(cnz0 - nbase);
-            const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + tid) * 16);
-            reinterpret_cast<uint4 *>(s_cmask)[2 * ci] = cm[0];
-            reinterpret_cast<uint4 *>(s_cmask)[2 * ci + 1] = cm[1];
-            reinterpret_cast<uint4 *>(s_cptr)[ci] = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + tid) * 16)[0];
+            reinterpret_cast<uint4 *>(s_cmask)[2 * ci] = cmk0;
+            reinterpret_cast<uint4 *>(s_cmask)[2 * ci + 1] = cmk1;
+            reinterpret_cast<uint4 *>(s_cptr)[ci] = cpt;
         }
     }
     __syncthreads();
