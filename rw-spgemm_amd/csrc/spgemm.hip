@@ -1756,17 +1756,25 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
             reinterpret_cast<uint4 *>(s_cptr)[ci] = cpt;
         }
     }
-    __syncthreads();
-    // pair (A tile j, B tile qi) at x = j * G3_QB + qi: no search for the A
-    // tile (B tile-rows hold at most G3_QB tiles); its slot index is the
-    // pair's rank SROW(7, j) + qi
+    // B tile columns of the row's (A tile j, B tile qi) pairs, xx = j * G3_QB
+    // + qi: those of the first 32 A tiles are loaded here, so that their
+    // latency overlaps the barrier (the rest after it)
     static_assert(G3_QB == 32, "pair index split by shifts");
-#pragma unroll 2
-    for (int xx = tid; xx < la * G3_QB; xx += G3_THREADS) {
-        const int j = xx >> 5, qi = xx & 31;
-        if (qi >= SROW(4, j)) continue;
+    static_assert(G3_LAMAX * G3_QB <= 8 * G3_THREADS, "eight pairs per thread");
+    int pj[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int xx = tid + G3_THREADS * t, j = xx >> 5, qi = xx & 31;
+        pj[t] = xx < la * G3_QB && qi < SROW(4, j) ? op.Bcol[SROW(3, j) + qi] : -1;
+    }
+    __syncthreads();
+    // the pair's slot index is its rank SROW(7, j) + qi
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int xx = tid + G3_THREADS * t, j = xx >> 5, qi = xx & 31;
+        const int J = t < 4 ? pj[t & 3] : xx < la * G3_QB && qi < SROW(4, j) ? op.Bcol[SROW(3, j) + qi] : -1;
+        if (J < 0) continue;
         const int x = SROW(7, j) + qi;
-        const int J = op.Bcol[SROW(3, j) + qi];
         if (direct) {
             s_pslot[x] = s_map[J - jlo];
         } else {
