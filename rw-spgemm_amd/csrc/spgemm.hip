@@ -1438,6 +1438,11 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
 // previous one is computed, and the global loads of the step after next
 // are in flight meanwhile.  A barrier precedes each staged K-step only.
 #define G3_LAMAX 64  // A tiles per tile-row (row table built by wave 0)
+// A C tile with at least G3_PADMIN nonzeros is followed by one unused
+// accumulator element, so that the starts of equally sized tiles (a
+// stencil's repeating 3 / 74 / 3 nonzeros) fall in different LDS banks; the
+// lanes of a wave update the same row of many such tiles at once.
+#define G3_PADMIN 16
 
 // Dynamic LDS layout of k_step3_rows, from the maxima over its rows.
 struct G3Layout {
@@ -1451,11 +1456,11 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
         // (int) and their slot ids (u8) of all the row's C tiles
         2u * 8u * (size_t)L.bnnz > 5u * G3_UMAX ? 2u * 8u * (size_t)L.bnnz : 5u * G3_UMAX,
         2u * 8u * (size_t)L.amax,         // 2 A values
-        0u,                               // 3 (C tile columns share 1)
+        (size_t)L.umax,                   // 3 accumulator pads before each C tile (u8)
         2u * (size_t)L.umax,              // 4 C tile offsets (u16)
         32u * (size_t)L.umax,             // 5 C masks
         16u * (size_t)L.umax,             // 6 C row offsets
-        28u * (size_t)L.lamax,            // 7 row table (7 ints per A tile)
+        20u * (size_t)L.lamax,            // 7 row table (5 ints per A tile)
         (size_t)L.lamax * (size_t)L.qb,   // 8 pair slots
         2u * 32u * (size_t)L.qb,          // 9 B masks
         2u * 16u * (size_t)L.qb,          // 10 B row offsets
@@ -1484,7 +1489,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 
 #define G3_LOAD(j, S)                                                                                         \
     do {                                                                                                      \
-        const int lb_ = SROW(4, j), q0_ = SROW(3, j);                                                         \
+        const int lb_ = G3_LB(j), q0_ = SROW(3, j);                                                           \
         /* Every lane loads from an in-bounds address (clamped; the lanes past  \
            the stage's data load a valid element and the store skips them), so \
            the loads are straight-line code and the store of this stage, two   \
@@ -1496,7 +1501,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             S##d0 = m_.x; S##d1 = m_.y; S##d2 = o_.x; S##d3 = o_.y;                                           \
             S##e0 = op.Bnnz[q0_ + min(lane, lb_)]; /* made relative at the store */                           \
         } else if (wave == 1) {                                                                               \
-            const int pan_ = SROW(2, j), pa0_ = SROW(1, j); /* A tiles are never empty */                    \
+            const int pan_ = G3_PAN(j), pa0_ = SROW(1, j); /* A tiles are never empty */                    \
             S##d0 = op.Aval[pa0_ + min(lane, pan_ - 1)];                                                      \
             S##d1 = op.Aval[pa0_ + min(lane + 64, pan_ - 1)];                                                 \
             S##d2 = op.Aval[pa0_ + min(lane + 128, pan_ - 1)];                                                \
@@ -1505,8 +1510,8 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             S##e0 = op.Amask[x_];                                                                             \
             S##e1 = op.Aptr16[x_];                                                                            \
         } else {                                                                                              \
-            const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
-            const int pb0_ = pbn_ > 0 ? SROW(5, j) : 0, pbl_ = max(pbn_ - 1, 0);                              \
+            const int pbn_ = SROW(5, j), b_ = (wave - 2) * 64 + lane;                                         \
+            const int pb0_ = pbn_ > 0 ? SROW(4, j) : 0, pbl_ = max(pbn_ - 1, 0);                              \
             S##d0 = op.Bval[pb0_ + min(b_, pbl_)];                                                            \
             S##d1 = op.Bval[pb0_ + min(b_ + 128, pbl_)];                                                      \
             S##d2 = op.Bval[pb0_ + min(b_ + 256, pbl_)];                                                      \
@@ -1516,11 +1521,11 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 
 #define G3_STORE(j, S, buf)                                                                                   \
     do {                                                                                                      \
-        const int lb_ = SROW(4, j);                                                                           \
+        const int lb_ = G3_LB(j);                                                                             \
         if (wave == 0) {                                                                                      \
             if (lane < 2 * lb_) reinterpret_cast<double2 *>(s_bm + (buf) * qb * 16)[lane] = make_double2(S##d0, S##d1); \
             if (lane < lb_) reinterpret_cast<double2 *>(s_bp + (buf) * qb * 16)[lane] = make_double2(S##d2, S##d3); \
-            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = (unsigned short)((int)S##e0 - SROW(5, j));     \
+            if (lane <= lb_) s_bnnz[(buf) * (qb + 1) + lane] = (unsigned short)((int)S##e0 - SROW(4, j));     \
             /* nonempty-row bits of this half tile, OR-ed with the partner half */                            \
             const unsigned wd_[4] = {(unsigned)__double2loint(S##d0), (unsigned)__double2hiint(S##d0),        \
                                      (unsigned)__double2loint(S##d1), (unsigned)__double2hiint(S##d1)};       \
@@ -1533,7 +1538,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
             nz_ |= __shfl_xor(nz_, 1, 64);                                                                    \
             if (!(lane & 1) && lane < 2 * lb_) s_bnz[(buf) * qb + (lane >> 1)] = (unsigned short)nz_;         \
         } else if (wave == 1) {                                                                               \
-            const int pan_ = SROW(2, j);                                                                      \
+            const int pan_ = G3_PAN(j);                                                                      \
             double *av_ = s_aval + (buf) * amax;                                                              \
             if (lane < pan_) av_[lane] = S##d0;                                                               \
             if (lane + 64 < pan_) av_[lane + 64] = S##d1;                                                     \
@@ -1544,7 +1549,7 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
                 s_ap[buf][lane] = (unsigned char)S##e1;                                                       \
             }                                                                                                 \
         } else {                                                                                              \
-            const int pbn_ = SROW(6, j), b_ = (wave - 2) * 64 + lane;                                         \
+            const int pbn_ = SROW(5, j), b_ = (wave - 2) * 64 + lane;                                         \
             double *bv_ = s_bval + (buf) * bnnzs;                                                             \
             if (b_ < pbn_) bv_[b_] = S##d0;                                                                   \
             if (b_ + 128 < pbn_) bv_[b_ + 128] = S##d1;                                                       \
@@ -1553,11 +1558,15 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
         }                                                                                                     \
     } while (0)
 
-// Row table entry k of A tile j (k = 1..7): 1 its first value, 2 its entry
-// count, 3 the first tile and 4 the tile count of B tile-row K, 5 and 6 the
-// first value and value count of that B tile-row (for a single-entry tile:
-// 5 = its row | column << 8), 7 the rank of its first (A tile, B tile) pair.
+// Row table entry k of A tile j (k = 1..5): 1 its first value, 2 its entry
+// count | the tile count of B tile-row K << 9 | the rank of its first (A
+// tile, B tile) pair << 15 (at most 256, 32 and 63 * 32), 3 the first tile
+// of B tile-row K, 4 and 5 the first value and value count of that B
+// tile-row (for a single-entry tile: 4 = its row | column << 8).
 #define SROW(k, j) s_row[((k) - 1) * lamax + (j)]
+#define G3_PAN(j) (SROW(2, j) & 511)
+#define G3_LB(j) ((SROW(2, j) >> 9) & 63)
+#define G3_PB(j) (SROW(2, j) >> 15)
 
 // Wave owning C row r: rows r = 4i + c belong to wave (2, 0, 1, 3)[c], so the
 // rows whose single-entry tiles a stencil produces (0 and 15) fall on the
@@ -1574,7 +1583,7 @@ __device__ __forceinline__ int g3_rbase(int w) { return (0xC9 >> (2 * w)) & 3; }
 // row k loads a valid element and computes nothing).
 #define G3_L1(j)                                                                      \
     do {                                                                              \
-        const int rk_ = SROW(5, j), nq_ = SROW(4, j);                                 \
+        const int rk_ = SROW(4, j), nq_ = G3_LB(j);                                  \
         const int q_ = max(SROW(3, j) + min(lane, nq_ - 1), 0);                       \
         const long long x_ = (long long)q_ * 16 + (rk_ >> 8);                         \
         sm_m = op.Bmask[x_];                                                          \
@@ -1583,16 +1592,16 @@ __device__ __forceinline__ int g3_rbase(int w) { return (0xC9 >> (2 * w)) & 3; }
     } while (0)
 #define G3_L2(j)                                                                      \
     do {                                                                              \
-        const int nb_ = __popc(sm_m), nq_ = SROW(4, j), rr_ = SROW(5, j) & 15;        \
+        const int nb_ = __popc(sm_m), nq_ = G3_LB(j), rr_ = SROW(4, j) & 15;         \
         const int pos_ = (int)(sm_n + sm_p), bs_ = nb_ > 0 ? pos_ : (int)sm_n;        \
         sv_0 = op.Bval[bs_];                                                          \
         sv_1 = op.Bval[bs_ + min(1, max(nb_ - 1, 0))];                                \
         sv_a = op.Aval[SROW(1, j)];                                                   \
         sv_m = sm_m;                                                                  \
         sv_p = pos_;                                                                  \
-        const int sl_ = nq_ > 0 ? s_pslot[SROW(7, j) + min(lane, nq_ - 1)] : 0;       \
+        const int sl_ = nq_ > 0 ? s_pslot[G3_PB(j) + min(lane, nq_ - 1)] : 0;         \
         sv_cm = s_cmask[sl_ * 16 + rr_];                                              \
-        sv_cb = s_coff[sl_] + s_cptr[sl_ * 16 + rr_];                                 \
+        sv_cb = s_coff[sl_] + s_pad[sl_] + s_cptr[sl_ * 16 + rr_];                    \
     } while (0)
 // K-step j = (row rr, column k): row k of the lane's B tile into row rr of
 // the pair's C tile, ascending column (values past the second from memory);
@@ -1600,7 +1609,7 @@ __device__ __forceinline__ int g3_rbase(int w) { return (0xC9 >> (2 * w)) & 3; }
 #define G3_SINGLE(j)                                                                  \
     do {                                                                              \
         unsigned bm_ = sv_m;                                                          \
-        if (lane < SROW(4, j) && bm_) {                                               \
+        if (lane < G3_LB(j) && bm_) {                                                 \
             double *ac_ = s_acc + sv_cb;                                              \
             int jb_ = 0;                                                              \
             while (bm_) {                                                             \
@@ -1641,6 +1650,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     int *s_ccol = reinterpret_cast<int *>(g3_smem + off[1]);  // setup only: the B value area
     unsigned char *s_cidx = g3_smem + off[1] + 4 * G3_UMAX;    // setup only
     unsigned short *s_coff = reinterpret_cast<unsigned short *>(g3_smem + off[4]);  // < G3_NNZMAX
+    unsigned char *s_pad = g3_smem + off[3];  // accumulator pads before each C tile
     unsigned short *s_cmask = reinterpret_cast<unsigned short *>(g3_smem + off[5]);
     unsigned char *s_cptr = g3_smem + off[6];
     int *s_row = reinterpret_cast<int *>(g3_smem + off[7]);
@@ -1651,7 +1661,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     unsigned short *s_bnz = reinterpret_cast<unsigned short *>(g3_smem + off[12]);
     __shared__ unsigned short s_am[2][16];
     __shared__ unsigned char s_ap[2][16];
-    __shared__ int s_wne[G3_THREADS / 64];
+    __shared__ int s_wne[G3_THREADS / 64], s_wbig[G3_THREADS / 64];
     __shared__ unsigned long long s_stg, s_own[G3_THREADS / 64];  // staged steps; singles per wave
     static_assert(G3_UMAX <= G3_THREADS, "one C tile per thread in the setup");
     const int tid = threadIdx.x;
@@ -1666,20 +1676,24 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     // at their rank among the nonempty tiles: a task is active only when
     // one of its products exists, so no task lands in an empty C tile.
     // (their masks and row offsets are loaded here, stored after the barrier)
-    int ccol = 0, cnz0 = 0;
+    int ccol = 0, cnz0 = 0, cnz1 = 0;
     bool ne = false;
     uint4 cmk0 = make_uint4(0u, 0u, 0u, 0u), cmk1 = cmk0, cpt = cmk0;
     if (tid < u) {
         ccol = Ccol[c0 + tid];
         cnz0 = Cnnz[c0 + tid];
-        ne = Cnnz[c0 + tid + 1] > cnz0;
+        cnz1 = Cnnz[c0 + tid + 1];
+        ne = cnz1 > cnz0;
         const uint4 *cm = reinterpret_cast<const uint4 *>(Cmask + (long long)(c0 + tid) * 16);
         cmk0 = cm[0];
         cmk1 = cm[1];
         cpt = reinterpret_cast<const uint4 *>(Cptr16 + (long long)(c0 + tid) * 16)[0];
     }
-    const unsigned long long neb = __ballot(ne);
-    if (lane == 0) s_wne[wave] = __popcll(neb);
+    const unsigned long long neb = __ballot(ne), bgb = __ballot(cnz1 - cnz0 >= G3_PADMIN);
+    if (lane == 0) {
+        s_wne[wave] = __popcll(neb);
+        s_wbig[wave] = __popcll(bgb);
+    }
     // row table: one entry per A tile of the tile-row (wave 0, la <= 64)
     if (wave == 0) {
         int lb = 0, pan = 0, rk = 0;
@@ -1699,14 +1713,12 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
             }
             lb = q1 - q0;
             SROW(1, lane) = pa0;
-            SROW(2, lane) = pan;
             SROW(3, lane) = q0;
-            SROW(4, lane) = lb;
-            SROW(5, lane) = pan == 1 ? rk : pb0;
-            SROW(6, lane) = pb1 - pb0;
+            SROW(4, lane) = pan == 1 ? rk : pb0;
+            SROW(5, lane) = pb1 - pb0;
         }
         const int incl = tsg_wave_incl_scan(lb);
-        if (lane < la) SROW(7, lane) = incl - lb;
+        if (lane < la) SROW(2, lane) = pan | (lb << 9) | ((incl - lb) << 15);
         const unsigned long long stg = __ballot(pan > 1);
         unsigned long long ow[G3_THREADS / 64];
 #pragma unroll
@@ -1731,11 +1743,17 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     double sv_0 = 0.0, sv_1 = 0.0, sv_a = 0.0;
     int sA = g3_pop(own), sB = g3_pop(own);  // this wave's next two single-entry steps
     if (sA < 64) G3_L1(sA);
-    int ci = __popcll(neb & ((1ull << lane) - 1ull));
-    for (int w = 0; w < wave; w++) ci += s_wne[w];
-    int un = 0;
+    int ci = __popcll(neb & ((1ull << lane) - 1ull)), pad = __popcll(bgb & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; w++) {
+        ci += s_wne[w];
+        pad += s_wbig[w];
+    }
+    int un = 0, npad = 0;
 #pragma unroll
-    for (int w = 0; w < G3_THREADS / 64; w++) un += s_wne[w];
+    for (int w = 0; w < G3_THREADS / 64; w++) {
+        un += s_wne[w];
+        npad += s_wbig[w];
+    }
     // C tile slot of every (A tile, B tile) pair of the tile-row.  When the
     // row's C tile columns span at most the accumulator's bytes, the
     // accumulator area first holds a direct map column -> slot (every pair's
@@ -1751,6 +1769,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         if (direct) s_map[ccol - jlo] = (unsigned char)ci;
         if (ne) {
             s_coff[ci] = (unsigned short)(cnz0 - nbase);
+            s_pad[ci] = (unsigned char)pad;
             reinterpret_cast<uint4 *>(s_cmask)[2 * ci] = cmk0;
             reinterpret_cast<uint4 *>(s_cmask)[2 * ci + 1] = cmk1;
             reinterpret_cast<uint4 *>(s_cptr)[ci] = cpt;
@@ -1765,16 +1784,16 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int xx = tid + G3_THREADS * t, j = xx >> 5, qi = xx & 31;
-        pj[t] = xx < la * G3_QB && qi < SROW(4, j) ? op.Bcol[SROW(3, j) + qi] : -1;
+        pj[t] = xx < la * G3_QB && qi < G3_LB(j) ? op.Bcol[SROW(3, j) + qi] : -1;
     }
     __syncthreads();
-    // the pair's slot index is its rank SROW(7, j) + qi
+    // the pair's slot index is its rank G3_PB(j) + qi
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const int xx = tid + G3_THREADS * t, j = xx >> 5, qi = xx & 31;
-        const int J = t < 4 ? pj[t & 3] : xx < la * G3_QB && qi < SROW(4, j) ? op.Bcol[SROW(3, j) + qi] : -1;
+        const int J = t < 4 ? pj[t & 3] : xx < la * G3_QB && qi < G3_LB(j) ? op.Bcol[SROW(3, j) + qi] : -1;
         if (J < 0) continue;
-        const int x = SROW(7, j) + qi;
+        const int x = G3_PB(j) + qi;
         if (direct) {
             s_pslot[x] = s_map[J - jlo];
         } else {
@@ -1787,7 +1806,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
         }
     }
     __syncthreads();  // the map area becomes the accumulator
-    for (int i = tid; i < nrow; i += G3_THREADS) s_acc[i] = 0.0;
+    for (int i = tid; i < nrow + npad; i += G3_THREADS) s_acc[i] = 0.0;
     if (sA < 64) G3_L2(sA);
     if (sB < 64) G3_L1(sB);
     if (fa < 64) G3_STORE(fa, R0_, 0);
@@ -1795,7 +1814,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 
     // compute staged K-step j from buffer buf: the tasks of this wave's rows
     auto compute = [&](int j, int buf) {
-        const int nq = SROW(4, j), pbase = SROW(7, j);
+        const int nq = G3_LB(j), pbase = G3_PB(j);
         const int ncand = nq * 4;
         const unsigned short *sbz = s_bnz + buf * qb;
         // candidate tasks (B tile qi, A row rb + 4i) meeting at least one
@@ -1836,7 +1855,7 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
             const int qi = code >> 4, rr = code & 15;
             const int slot = s_pslot[pbase + qi];
             const unsigned cm = s_cmask[slot * 16 + rr];
-            const int cbase = s_coff[slot] + s_cptr[slot * 16 + rr];
+            const int cbase = s_coff[slot] + s_pad[slot] + s_cptr[slot * 16 + rr];
             const unsigned short *bm16 = sbm + qi * 16;
             const unsigned char *bp16 = sbp + qi * 16;
             const double *bvt = sbv + sbnnz[qi];
@@ -1886,17 +1905,22 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
     }
 #undef G3_ITER
     __syncthreads();  // every wave's rows are complete
-    // write out: values are already in final order; columns from the masks
-    for (int i = tid; i < nrow; i += G3_THREADS) Cval[(long long)nbase + i] = s_acc[i];
+    // write out, one C tile row at a time: values from the padded
+    // accumulator, columns from the masks
     for (int x = tid; x < un * 16; x += G3_THREADS) {
         const int slot = x >> 4;
         unsigned cm = s_cmask[x];
-        unsigned char *oc = Ccol8 + nbase + s_coff[slot] + s_cptr[x];
+        const int fo = s_coff[slot] + s_cptr[x];
+        const double *av = s_acc + fo + s_pad[slot];
+        unsigned char *oc = Ccol8 + nbase + fo;
+        double *ov = Cval + (long long)nbase + fo;
         int jj = 0;
         while (cm) {
             const int bb = 31 - __clz(cm);
             cm &= ~(1u << bb);
-            oc[jj++] = (unsigned char)(15 - bb);
+            oc[jj] = (unsigned char)(15 - bb);
+            ov[jj] = av[jj];
+            jj++;
         }
     }
 }
@@ -1905,8 +1929,9 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 // Step-3 class of every C tile-row: 1 = row-wise kernel (its C tiles, C
 // nonzeros, A tiles and B tile-rows are within the G3 caps), 0 = tile-parallel
 // kernel, 2 = nothing to compute.  out[0] += class-0 rows; out[1..6] = max
-// over class-1 rows of (nonempty C tiles, C nonzeros, A tiles, B tiles per B
-// tile-row, B nonzeros per B tile-row, A tile nonzeros) for the LDS layout.
+// over class-1 rows of (nonempty C tiles, C nonzeros plus accumulator pads,
+// A tiles, B tiles per B tile-row, B nonzeros per B tile-row, A tile
+// nonzeros) for the LDS layout.
 // 16 lanes per tile-row (16 tile-rows per block): the lanes walk the row's A
 // tiles and C tiles in strides of 16, so the dependent loads of different A
 // tiles are in flight together; one set of atomics per block.
@@ -1942,13 +1967,20 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
                     lbn = max(lbn, bn);
                     lan = max(lan, op.Annz[p + 1] - op.Annz[p]);
                 }
-                // nonempty C tiles (the row-wise kernel's slots)
-                for (int t = c0 + r; t < c1; t += 16) ne += Cnnz[t + 1] > Cnnz[t];
+                // nonempty C tiles (the row-wise kernel's slots) and those
+                // followed by an accumulator pad
+                int nbig = 0;
+                for (int t = c0 + r; t < c1; t += 16) {
+                    const int tn = Cnnz[t + 1] - Cnnz[t];
+                    ne += tn > 0;
+                    nbig += tn >= G3_PADMIN;
+                }
                 ok = g16_max(bad) == 0;
                 ne = tsg_group16_sum(ne);
+                nbig = tsg_group16_sum(nbig);
                 lq = g16_max(lq); lbn = g16_max(lbn); lan = g16_max(lan);
                 if (ok) {
-                    mu = ne; mn = nrow; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
+                    mu = ne; mn = nrow + nbig; ml = a1 - a0; mq = lq; mb = lbn; ma = lan;
                 }
             }
             k = ok ? 1 : 0;
