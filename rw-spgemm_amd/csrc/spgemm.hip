@@ -429,7 +429,9 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     __shared__ unsigned char s_rk[G2_PCHUNK];
     __shared__ int s_wsg[G2_THREADS / 64];
     __shared__ int s_tot;
-    __shared__ unsigned short s_bst[G2_THREADS][16];
+    // B masks of each thread's pair, word-major (word i = rows 2i (low half)
+    // and 2i+1), so that a wave's lanes read and write consecutive banks
+    __shared__ unsigned s_bst[8][G2_THREADS];
     const int tid = threadIdx.x;
     // the grid is an upper bound: the first *nseg blocks (spread evenly over
     // the XCDs) take the segments, each XCD a contiguous range
@@ -450,7 +452,6 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
     }
     for (int i = tid; i < u * G2_MSTRIDE; i += G2_THREADS) s_m[i] = 0u;
     const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
-    unsigned short *bst = s_bst[tid];
     __shared__ int s_wsum[G2_THREADS / 64];
     for (int pc = a0; pc < a1; pc += G2_PCHUNK) {
         const int np = min(G2_PCHUNK, a1 - pc);
@@ -543,8 +544,14 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
             }
             const uint4 *bm4 = reinterpret_cast<const uint4 *>(op.Bmask + (long long)q * 16);
             const uint4 bA = bm4[0], bB = bm4[1];
-            *reinterpret_cast<uint4 *>(bst) = bA;
-            *reinterpret_cast<uint4 *>(bst + 8) = bB;
+            s_bst[0][tid] = bA.x;
+            s_bst[1][tid] = bA.y;
+            s_bst[2][tid] = bA.z;
+            s_bst[3][tid] = bA.w;
+            s_bst[4][tid] = bB.x;
+            s_bst[5][tid] = bB.y;
+            s_bst[6][tid] = bB.z;
+            s_bst[7][tid] = bB.w;
             const uint4 *am4 = reinterpret_cast<const uint4 *>(op.Amask + (long long)(pc + s_pid[p]) * 16);
             const uint4 aA = am4[0], aB = am4[1];
             const unsigned aw[8] = {aA.x, aA.y, aA.z, aA.w, aB.x, aB.y, aB.z, aB.w};
@@ -556,7 +563,8 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
                 while (am) {
                     const int b = 31 - __clz(am);
                     am &= ~(1u << b);
-                    m |= bst[15 - b];
+                    const int k = 15 - b;
+                    m |= (s_bst[k >> 1][tid] >> ((k & 1) * 16)) & 0xffffu;
                 }
                 atomicOr(&s_m[slot * G2_MSTRIDE + (r >> 1)], m << ((r & 1) ? 0 : 16));
             }
