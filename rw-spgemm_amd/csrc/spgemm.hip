@@ -517,6 +517,8 @@ k_step2_rows(TileOperands op, const int4 *__restrict__ segs, const int *__restri
         if (tid == G2_THREADS - 1) s_tot = woff + incl;
         __syncthreads();
         const int total = s_tot;
+        // two pairs per round, so that both pairs' B tile loads are in flight
+#pragma unroll 2
         for (int x = tid; x < total; x += G2_THREADS) {
             int p = 0;
 #pragma unroll
