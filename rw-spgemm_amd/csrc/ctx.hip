@@ -260,14 +260,6 @@ int tsg_scratch(tsg_ctx *c, size_t bytes, void **p) {
     return TSG_OK;
 }
 
-int tsg_read_i64_pair(tsg_ctx *c, const void *dptr, long long out[2]) {
-    TSG_HIP(hipMemcpyAsync(c->pinned, dptr, 16, hipMemcpyDeviceToHost, c->stream));
-    TSG_HIP(hipStreamSynchronize(c->stream));
-    out[0] = c->pinned[0];
-    out[1] = c->pinned[1];
-    return TSG_OK;
-}
-
 int tsg_read_scalar_i64(tsg_ctx *c, const void *dptr, int is64, long long *out) {
     TSG_HIP(hipMemcpyAsync(c->pinned, dptr, is64 ? 8 : 4, hipMemcpyDeviceToHost, c->stream));
     TSG_HIP(hipStreamSynchronize(c->stream));

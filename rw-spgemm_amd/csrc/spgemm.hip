@@ -44,7 +44,8 @@ template <bool FILL, int BMW>
 __global__ void __launch_bounds__(64 * S1_WAVES)
 k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
         const int *__restrict__ Bptr, const int *__restrict__ Bcol,
-        int *__restrict__ Cptr_or_cnt, int *__restrict__ Ccol, int *__restrict__ Crow, unsigned *__restrict__ bmstore) {
+        int *__restrict__ Cptr_or_cnt, int *__restrict__ Ccol, int *__restrict__ Crow, unsigned *__restrict__ bmstore,
+        int *__restrict__ rowspan) {
     __shared__ unsigned s_bm[S1_WAVES][BMW];
     __shared__ int s_pre[S1_WAVES][64];
     __shared__ int s_b0[S1_WAVES][64];
@@ -199,7 +200,11 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
         }
         tsg_wave_sync();
     }
-    if (!FILL && lane == 0) Cptr_or_cnt[I] = total;
+    if (!FILL && lane == 0) {
+        Cptr_or_cnt[I] = total;
+        // the candidates are exactly C's tiles, so this is the row's C tile-column span
+        if (rowspan) rowspan[I] = total > 0 ? jmax - jmin + 1 : 0;
+    }
 }
 
 // ------------------------------------------------------------------ pairs
@@ -391,17 +396,24 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 
 // Most C tiles of a tile-row (capped at the step-2 segment size), the
 // widest tile-column span of a tile-row, and the widest span of at most
-// G2_MAP_MAX: out[0], out[1], out[2] (atomic maxima).
+// G2_MAP_MAX: out[0], out[1], out[2] (atomic maxima), from the step-1 count
+// pass's per-row tile counts and spans.
 __global__ void __launch_bounds__(256)
-k_tilerow_stats(const int *__restrict__ Cptr, const int *__restrict__ Ccol, int tilem, int *__restrict__ out) {
+k_row_stats(const int *__restrict__ cnt, const int *__restrict__ span, int tilem, int *__restrict__ out) {
     const int I = blockIdx.x * 256 + threadIdx.x;
-    if (I >= tilem) return;
-    const int c0 = Cptr[I], c1 = Cptr[I + 1];
-    if (c1 > c0) {
-        const int span = Ccol[c1 - 1] - Ccol[c0] + 1;
-        atomicMax(&out[0], min(c1 - c0, G2_UMAX));
-        atomicMax(&out[1], span);
-        if (span <= G2_MAP_MAX) atomicMax(&out[2], span);
+    int nu = 0, sp = 0, ms = 0;
+    if (I < tilem && cnt[I] > 0) {
+        nu = min(cnt[I], G2_UMAX);
+        sp = span[I];
+        ms = sp <= G2_MAP_MAX ? sp : 0;
+    }
+    nu = tsg_wave_max(nu);
+    sp = tsg_wave_max(sp);
+    ms = tsg_wave_max(ms);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], nu);
+        atomicMax(&out[1], sp);
+        atomicMax(&out[2], ms);
     }
 }
 
@@ -2029,7 +2041,7 @@ extern "C" int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *A, const t
     if (A->tilem > 0) {
         hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
                            ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
-                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr);
+                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr);
         TSG_LAUNCH_CHECK();
     }
     long long tot = 0;
@@ -2127,7 +2139,7 @@ extern "C" int tsg_spgemm_count_tiles_noalloc(tsg_ctx *ctx, const tsg_dtiles *A,
     if (A->tilem > 0) {
         hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
                            ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
-                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr);
+                           (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr);
         TSG_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_noalloc_partial, dim3(nparts), dim3(256), 0, ctx->stream, (const int *)cnt, A->tilem, part);
@@ -2172,22 +2184,22 @@ static size_t keep_bytes(long long tilem, int words) {
 }
 
 static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C, bool fill,
-                        int words, unsigned *bmstore) {
+                        int words, unsigned *bmstore, int *rowspan = nullptr) {
     if (C->tilem <= 0) return TSG_OK;
     const dim3 g(tsg_div_up(C->tilem, S1_WAVES)), blk(64 * S1_WAVES);
     const bool wide = words == 2048;
     if (!fill && !wide)
         hipLaunchKernelGGL((k_step1<false, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore);
+                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
     else if (!fill)
         hipLaunchKernelGGL((k_step1<false, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore);
+                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
     else if (!wide)
         hipLaunchKernelGGL((k_step1<true, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore);
+                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore, (int *)nullptr);
     else
         hipLaunchKernelGGL((k_step1<true, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore);
+                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore, (int *)nullptr);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
@@ -2375,14 +2387,14 @@ struct SpgemmTemps {
     unsigned *keep = nullptr;
     void *scr = nullptr, *wbuf = nullptr, *w3buf = nullptr;
     unsigned char *hflag = nullptr;
-    int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr;
+    int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr, *span = nullptr;
     int2 *hitems = nullptr;
     hipEvent_t ev[6] = {};
     int nev = 0;
 };
 
 static void free_temps(tsg_ctx *ctx, SpgemmTemps &t) {
-    void *ps[] = {t.keep, t.scr, t.wbuf, t.w3buf, t.hflag, t.hcnt, t.hinfo, t.hcur, t.hitems};
+    void *ps[] = {t.keep, t.scr, t.wbuf, t.w3buf, t.hflag, t.hcnt, t.hinfo, t.hcur, t.hitems, t.span};
     for (void *p : ps) tsg_free(ctx, p);
     for (int i = 0; i < t.nev; i++) (void)hipEventDestroy(t.ev[i]);
     t = SpgemmTemps();
@@ -2437,13 +2449,41 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     P.s1w = tsg_s1_words(B, tune);
     const size_t kb = keep_bytes(C->tilem, P.s1w);
     if (kb) TSG_TRY(tsg_alloc(ctx, (void **)&T.keep, kb));
-    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep));
     TSG_TRY(tsg_alloc(ctx, &T.scr, 64));
     void *scr = T.scr;
     unsigned long long *bins = (unsigned long long *)scr;
+    int *st2 = reinterpret_cast<int *>(static_cast<char *>(scr) + 32);  // step-2 sizing maxima
     TSG_HIP(hipMemsetAsync(scr, 0, 64, s));
+    TSG_TRY(tsg_alloc_t(ctx, &T.span, (size_t)C->tilem + 1));
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep, T.span));
+    if (C->tilem > 0) {
+        hipLaunchKernelGGL(k_row_stats, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, (const int *)C->tile_ptr,
+                           (const int *)T.span, C->tilem, st2);
+        TSG_LAUNCH_CHECK();
+    }
+    // C's tile-row offsets; their total and the step-2 sizing maxima come
+    // back in one read-back
     long long numtileC = 0;
-    TSG_TRY(tsg_scan_i32(ctx, C->tile_ptr, C->tile_ptr, C->tilem, &numtileC));
+    int smax[4] = {0, 0, 0, 0};
+    {
+        const long long np1 = tsg_scan_partial_count(C->tilem);
+        void *part1v;
+        TSG_TRY(tsg_scratch(ctx, (size_t)np1 * sizeof(long long), &part1v));
+        long long *part1 = (long long *)part1v;
+        TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, part1));
+        long long *pin = ctx->pinned;
+        TSG_HIP(hipMemcpyAsync(pin, part1 + np1 - 1, sizeof(long long), hipMemcpyDeviceToHost, s));
+        TSG_HIP(hipMemcpyAsync(pin + 1, st2, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        TSG_HIP(hipStreamSynchronize(s));
+        numtileC = pin[0];
+        if (numtileC > 0x7fffffffLL) {
+            tsg_set_error("prefix sum total %lld exceeds int32", numtileC);
+            return TSG_ERR_OVERFLOW;
+        }
+        memcpy(smax, pin + 1, sizeof(smax));
+    }
+    tsg_free(ctx, T.span);
+    T.span = nullptr;
     P.numtileC = numtileC;
     C->numtile = (int)numtileC;
     TSG_TRY(alloc_c_tiles(ctx, C, numtileC));
@@ -2461,13 +2501,7 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     P.s2u = G2_UMAX;
     P.s2map = 0;
     if (numtileC > 0) {
-        int *st2 = reinterpret_cast<int *>(static_cast<char *>(scr) + 32);  // zeroed with scr
-        hipLaunchKernelGGL(k_tilerow_stats, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, C->tile_ptr,
-                           C->tile_colidx, C->tilem, st2);
-        TSG_LAUNCH_CHECK();
-        long long v[2] = {0, 0};
-        TSG_TRY(tsg_read_i64_pair(ctx, st2, v));
-        const int umax = (int)(v[0] & 0xffffffffLL), span = (int)(v[0] >> 32), mspan = (int)(v[1] & 0xffffffffLL);
+        const int umax = smax[0], span = smax[1], mspan = smax[2];
         P.s2u = std::max(1, umax);
         P.s2map = umax <= 256 ? mspan : 0;
         P.s3pk = span < G3E_PK_SPAN ? 1 : 0;
@@ -2480,27 +2514,46 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
                            C->tile_nnz, numtileC, tune.dense_acc_min_nnz, bins);
         TSG_LAUNCH_CHECK();
     }
+    // C's tile offsets, then the step-3 row classes (which need only the
+    // device offsets); C's nonzero count and the class maxima come back in
+    // one read-back
     long long nnzC = 0;
-    TSG_TRY(tsg_scan_i32(ctx, C->tile_nnz, C->tile_nnz, numtileC, &nnzC));
+    const long long np2 = tsg_scan_partial_count(numtileC);
+    void *part2v;
+    TSG_TRY(tsg_scratch(ctx, (size_t)np2 * sizeof(long long), &part2v));
+    long long *part2 = (long long *)part2v;
+    TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, numtileC, part2));
+    if (timed) TSG_HIP(hipEventRecord(ev[2], s));
+    const bool classes = !P.all && numtileC > 0;
+    if (classes) {
+        TSG_HIP(hipMemsetAsync(w.cls_out, 0, 8 * sizeof(int), s));
+        TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
+    }
+    {
+        long long *pin = ctx->pinned;  // [0] the total, [1..4] the eight class words
+        TSG_HIP(hipMemcpyAsync(pin, part2 + np2 - 1, sizeof(long long), hipMemcpyDeviceToHost, s));
+        if (classes) TSG_HIP(hipMemcpyAsync(pin + 1, w.cls_out, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+        TSG_HIP(hipStreamSynchronize(s));
+        nnzC = pin[0];
+        if (nnzC > 0x7fffffffLL) {
+            tsg_set_error("prefix sum total %lld exceeds int32", nnzC);
+            return TSG_ERR_OVERFLOW;
+        }
+        if (classes) {
+            const int *h = reinterpret_cast<const int *>(pin + 1);
+            P.s3_unfit = h[0] > 0;
+            P.s3_rows = h[1] > 0;
+            for (int i = 0; i < 6; i++) P.g3[i] = std::max(1, h[1 + i]);
+        }
+    }
     P.nnzC = nnzC;
     C->nnz = (int)nnzC;
-    if (timed) TSG_HIP(hipEventRecord(ev[2], s));
     // ---- step 3
     if (!symbolic) {
         TSG_TRY(tsg_alloc_t(ctx, &C->csr_col, (size_t)nnzC + 1));
         TSG_TRY(tsg_alloc_t(ctx, &C->val, (size_t)nnzC + 1));
     }
     if (nnzC > 0) {
-        if (!P.all) {
-            TSG_HIP(hipMemsetAsync(w.cls_out, 0, 8 * sizeof(int), s));
-            TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
-            int h[8];
-            TSG_HIP(hipMemcpyAsync(h, w.cls_out, sizeof(h), hipMemcpyDeviceToHost, s));
-            TSG_HIP(hipStreamSynchronize(s));
-            P.s3_unfit = h[0] > 0;
-            P.s3_rows = h[1] > 0;
-            for (int i = 0; i < 6; i++) P.g3[i] = std::max(1, h[1 + i]);
-        }
         if (!P.all && P.s3_unfit && P.s3_elem) {
             const SpgemmWork w3 = work_at(wbuf, C->tilem, numtileC, nnzC, nullptr);
             TSG_TRY(tsg_alloc(ctx, &T.w3buf, 16 * (size_t)w3.seg3_cap));

@@ -95,8 +95,6 @@ static inline int tsg_alloc_t(tsg_ctx *ctx, T **p, size_t count) {
 }
 int tsg_scratch(tsg_ctx *ctx, size_t bytes, void **p);
 int tsg_read_scalar_i64(tsg_ctx *ctx, const void *dptr, int is64, long long *out);
-// Two consecutive int64 values at dptr (one copy, one stream sync).
-int tsg_read_i64_pair(tsg_ctx *ctx, const void *dptr, long long out[2]);
 // Bytes the allocator reserved for a live block of ctx (0 if none).
 size_t tsg_alloc_size(const tsg_ctx *ctx, const void *p);
 
