@@ -76,6 +76,47 @@ def test_empty_operands(ctx):
     spgemm_check(ctx, a, zz, label="A * empty")
 
 
+def _coo(n, rows, cols, rng):
+    import scipy.sparse as sp
+    a = sp.csr_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    a.sum_duplicates()
+    a.sort_indices()
+    return ts.Csr.from_scipy(a)
+
+
+@pytest.mark.parametrize("case", ["all_single", "mixed", "empty_b_rows", "one_residue"])
+def test_row_kernel_single_entry_tiles(ctx, case):
+    """The row-wise numeric kernel's single-entry K-steps (an A tile with one
+    nonzero (r, k) is never staged: the wave owning C row r loads row k of
+    each B tile itself) on tile-rows made for it, all within the row-wise
+    kernel's caps: only single-entry tiles (no staged step at all, several
+    per owning wave back to back), single- and two-entry tiles mixed, single
+    entries whose B tile-row is empty, and entries in one row residue mod 4
+    only (every task and every single-entry step on one wave)."""
+    rng = np.random.default_rng(11)
+    n = 16 * 48
+    i = np.arange(n)
+    if case == "all_single":  # one entry per row, columns a permutation
+        rows, cols = i, rng.permutation(n)
+    elif case == "mixed":  # a random entry per row, and beside it every third row
+        c = rng.integers(0, n - 1, n)
+        two = i % 3 == 0
+        rows, cols = np.concatenate([i, i[two]]), np.concatenate([c, c[two] + 1])
+    elif case == "empty_b_rows":  # rows of tile-rows 36..47 empty, columns anywhere
+        live = i[i < 16 * 36]
+        rows = np.concatenate([live, live])
+        cols = np.concatenate([rng.integers(0, n, len(live)), rng.integers(16 * 36, n, len(live))])
+    else:  # rows 4q + 3 only, one or two entries each
+        live = i[i % 4 == 3]
+        c = rng.integers(0, n - 1, len(live))
+        two = rng.random(len(live)) < 0.5
+        rows, cols = np.concatenate([live, live[two]]), np.concatenate([c, c[two] + 1])
+    a = _coo(n, rows, cols, rng)
+    st = spgemm_check(ctx, a, label=f"single-entry tiles: {case}")
+    assert st.t_step3_rows_ms > 0, "the row-wise numeric kernel took no tile-row"
+    spgemm_check(ctx, a, t(a), label=f"single-entry tiles: {case}, AA^T")
+
+
 def test_wide_tilerow_uses_tile_parallel_step2(ctx):
     """A hub row touching 700 tile columns makes C tile-row 0 hold 700 tiles
     (> the 512 the row-wise step-2 kernel keeps in LDS), so that row goes
