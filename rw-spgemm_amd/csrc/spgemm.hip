@@ -398,22 +398,35 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
 // widest tile-column span of a tile-row, and the widest span of at most
 // G2_MAP_MAX: out[0], out[1], out[2] (atomic maxima), from the step-1 count
 // pass's per-row tile counts and spans.
-__global__ void __launch_bounds__(256)
+// A few large blocks stride over the rows, so that the atomics (one cache
+// line, serialised) are few.
+#define ROW_STATS_BLOCKS 64
+__global__ void __launch_bounds__(1024)
 k_row_stats(const int *__restrict__ cnt, const int *__restrict__ span, int tilem, int *__restrict__ out) {
-    const int I = blockIdx.x * 256 + threadIdx.x;
+    __shared__ int s_r[16][3];
     int nu = 0, sp = 0, ms = 0;
-    if (I < tilem && cnt[I] > 0) {
-        nu = min(cnt[I], G2_UMAX);
-        sp = span[I];
-        ms = sp <= G2_MAP_MAX ? sp : 0;
+    for (int I = blockIdx.x * 1024 + threadIdx.x; I < tilem; I += gridDim.x * 1024) {
+        if (cnt[I] > 0) {
+            const int s1 = span[I];
+            nu = max(nu, min(cnt[I], G2_UMAX));
+            sp = max(sp, s1);
+            if (s1 <= G2_MAP_MAX) ms = max(ms, s1);
+        }
     }
     nu = tsg_wave_max(nu);
     sp = tsg_wave_max(sp);
     ms = tsg_wave_max(ms);
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&out[0], nu);
-        atomicMax(&out[1], sp);
-        atomicMax(&out[2], ms);
+        s_r[wave][0] = nu;
+        s_r[wave][1] = sp;
+        s_r[wave][2] = ms;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int v = s_r[0][threadIdx.x];
+        for (int w = 1; w < 16; w++) v = max(v, s_r[w][threadIdx.x]);
+        atomicMax(&out[threadIdx.x], v);
     }
 }
 
@@ -1950,13 +1963,16 @@ k_step3_rows(TileOperands op, const int *__restrict__ Cptr, const int *__restric
 
 // Step-3 class of every C tile-row: 1 = row-wise kernel (its C tiles, C
 // nonzeros, A tiles and B tile-rows are within the G3 caps), 0 = tile-parallel
-// kernel, 2 = nothing to compute.  out[0] += class-0 rows; out[1..6] = max
+// kernel, 2 = nothing to compute.  Per block, part[8 * block + i]: i = 0 the
+// class-0 rows, i = 1..6 the maxima
 // over class-1 rows of (nonempty C tiles, C nonzeros plus accumulator pads,
 // A tiles, B tiles per B tile-row, B nonzeros per B tile-row, A tile
 // nonzeros) for the LDS layout.
 // 16 lanes per tile-row (16 tile-rows per block): the lanes walk the row's A
 // tiles and C tiles in strides of 16, so the dependent loads of different A
-// tiles are in flight together; one set of atomics per block.
+// tiles are in flight together.  k_row_class_reduce folds the blocks' partials
+// into out[0] (sum) and out[1..6] (maxima): no atomics, which on one cache
+// line would serialise across the thousands of blocks.
 __device__ __forceinline__ int g16_max(int v) {
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 16));
@@ -1964,7 +1980,7 @@ __device__ __forceinline__ int g16_max(int v) {
 }
 __global__ void __launch_bounds__(256)
 k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem,
-            unsigned char *__restrict__ cls, int *__restrict__ out, int none_fit) {
+            unsigned char *__restrict__ cls, int *__restrict__ part, int none_fit) {
     __shared__ int s_red[4][7];
     const int r = threadIdx.x & 15, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int I = blockIdx.x * 16 + (threadIdx.x >> 4);
@@ -2018,16 +2034,45 @@ k_row_class(TileOperands op, const int *__restrict__ Cptr, const int *__restrict
         s_red[wave][4] = mq; s_red[wave][5] = mb; s_red[wave][6] = ma;
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < 8) {
         const int i = threadIdx.x;
-        int v = s_red[0][i];
+        int v = 0;
+        if (i < 7) {
+            v = s_red[0][i];
 #pragma unroll
-        for (int w = 1; w < 4; w++) v = i == 0 ? v + s_red[w][i] : max(v, s_red[w][i]);
-        if (i == 0) {
-            if (v) atomicAdd(&out[0], v);
-        } else {
-            atomicMax(&out[i], v);
+            for (int w = 1; w < 4; w++) v = i == 0 ? v + s_red[w][i] : max(v, s_red[w][i]);
         }
+        part[8 * (long long)blockIdx.x + i] = v;
+    }
+}
+
+// out[0] = sum and out[1..6] = maxima of the nb blocks' k_row_class partials
+// (one block of 1024 threads); out[7] = 0.
+__global__ void __launch_bounds__(1024)
+k_row_class_reduce(const int *__restrict__ part, int nb, int *__restrict__ out) {
+    __shared__ int s_r[16][7];
+    int v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+        v[0] += part[8 * (long long)b];
+#pragma unroll
+        for (int i = 1; i < 7; i++) v[i] = max(v[i], part[8 * (long long)b + i]);
+    }
+    v[0] = tsg_wave_sum(v[0]);
+#pragma unroll
+    for (int i = 1; i < 7; i++) v[i] = tsg_wave_max(v[i]);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 7; i++) s_r[wave][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int i = threadIdx.x;
+        int r = 0;
+        if (i < 7) {
+            r = s_r[0][i];
+            for (int w = 1; w < 16; w++) r = i == 0 ? r + s_r[w][i] : max(r, s_r[w][i]);
+        }
+        out[i] = r;
     }
 }
 
@@ -2212,6 +2257,7 @@ struct SpgemmWork {
     int *segcnt;       // tilem+1 segment counts -> offsets (total at [tilem])
     unsigned char *rowcls;
     int *cls_out;
+    int *cls_part;  // k_row_class partials, 8 ints per block
     int4 *seg2;        // step-2 segments
     unsigned *bmstore; // step-1 bitmaps kept from the count pass (may be NULL)
     int4 *seg3;        // step-3 element segments
@@ -2231,7 +2277,7 @@ static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
 // Offsets of the work arrays inside one buffer; returns its size.  The
 // step-3 segments go last (the synchronous pipeline adds them only once
 // C's nonzero count is known).
-#define WORK_NARR 9
+#define WORK_NARR 10
 
 static size_t work_layout(long long tilem, long long numtile, long long nnz, size_t off[WORK_NARR],
                           int s1w = 256) {
@@ -2240,6 +2286,7 @@ static size_t work_layout(long long tilem, long long numtile, long long nnz, siz
         8 * (size_t)tsg_scan_partial_count(tilem), 4 * ((size_t)tilem + 1), (size_t)tilem + 1, 32,
         16 * (size_t)seg_bound(tilem, numtile, G2_UMAX, 0, 0),
         keep_bytes(tilem, s1w),  // step-1 bitmaps (enqueue-only pipeline)
+        32 * (size_t)((tilem + 15) / 16 + 1),  // row-class partials, 8 ints per block
         16 * (size_t)(tilem + nnz / (G3E_NNZ - 255) + 2)};
     size_t o = 0;
     for (int i = 0; i < WORK_NARR; i++) {
@@ -2263,7 +2310,8 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.cls_out = (int *)(b + off[5]);
     w.seg2 = (int4 *)(b + off[6]);
     w.bmstore = keep_bytes(tilem, s1w) ? (unsigned *)(b + off[7]) : nullptr;
-    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[8]);
+    w.cls_part = (int *)(b + off[8]);
+    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[9]);
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
     w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
     w.hflag = nullptr;
@@ -2324,9 +2372,12 @@ static int launch_step2(hipStream_t s, const TileOperands &op, const tsg_dtiles 
 }
 
 static int launch_row_class(hipStream_t s, const TileOperands &op, tsg_dtiles *C, unsigned char *rowcls,
-                            int *cls_out, int none_fit) {
-    hipLaunchKernelGGL(k_row_class, dim3(tsg_div_up(C->tilem, 16)), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz,
-                       C->tilem, rowcls, cls_out, none_fit);
+                            int *cls_part, int *cls_out, int none_fit) {
+    const int nb = (int)tsg_div_up(C->tilem, 16);
+    hipLaunchKernelGGL(k_row_class, dim3(nb), dim3(256), 0, s, op, C->tile_ptr, C->tile_nnz, C->tilem, rowcls,
+                       cls_part, none_fit);
+    TSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_row_class_reduce, dim3(1), dim3(1024), 0, s, (const int *)cls_part, nb, cls_out);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
@@ -2457,8 +2508,8 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     TSG_TRY(tsg_alloc_t(ctx, &T.span, (size_t)C->tilem + 1));
     TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep, T.span));
     if (C->tilem > 0) {
-        hipLaunchKernelGGL(k_row_stats, dim3(tsg_div_up(C->tilem, 256)), dim3(256), 0, s, (const int *)C->tile_ptr,
-                           (const int *)T.span, C->tilem, st2);
+        hipLaunchKernelGGL(k_row_stats, dim3(std::min<unsigned>(ROW_STATS_BLOCKS, tsg_div_up(C->tilem, 1024))),
+                           dim3(1024), 0, s, (const int *)C->tile_ptr, (const int *)T.span, C->tilem, st2);
         TSG_LAUNCH_CHECK();
     }
     // C's tile-row offsets; their total and the step-2 sizing maxima come
@@ -2526,8 +2577,7 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     if (timed) TSG_HIP(hipEventRecord(ev[2], s));
     const bool classes = !P.all && numtileC > 0;
     if (classes) {
-        TSG_HIP(hipMemsetAsync(w.cls_out, 0, 8 * sizeof(int), s));
-        TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
+        TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_part, w.cls_out, P.elem_all));
     }
     {
         long long *pin = ctx->pinned;  // [0] the total, [1..4] the eight class words
@@ -2664,7 +2714,7 @@ int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, 
     TSG_TRY(launch_step2(s, op, B, C, P, w));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, w.part2));
     if (C->nnz > 0) {
-        if (!P.all) TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_out, P.elem_all));
+        if (!P.all) TSG_TRY(launch_row_class(s, op, C, w.rowcls, w.cls_part, w.cls_out, P.elem_all));
         TSG_TRY(launch_step3(s, op, B, C, P, w));
     }
     return TSG_OK;
