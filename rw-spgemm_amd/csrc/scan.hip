@@ -57,18 +57,32 @@ k_scan_partials(long long *partial, long long nb) {
     if (threadIdx.x == 0) partial[nb] = carry;
 }
 
-template <typename OutT>
+// Each thread rescans its 16 consecutive counts; a whole, 16-byte aligned
+// run moves as four int4 loads and stores (the scalar form strides the
+// lanes 64 bytes apart and splits every cache line over 16 instructions).
 __global__ void __launch_bounds__(SCAN_THREADS)
-k_scan_down(const int *in, long long n, const long long *__restrict__ partial, OutT *out) {
+k_scan_down(const int *in, long long n, const long long *__restrict__ partial, int *out) {
+    static_assert(SCAN_ITEMS == 16, "four int4 per thread");
     const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    const bool vec = base + SCAN_ITEMS <= n && !(((uintptr_t)in | (uintptr_t)out) & 15);
     int v[SCAN_ITEMS];
+    if (vec) {
+        const int4 *in4 = reinterpret_cast<const int4 *>(in + base);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int4 t = in4[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; j++) v[j] = base + j < n ? in[base + j] : 0;
+    }
     long long s = 0;
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-        long long i = base + j;
-        v[j] = i < n ? in[i] : 0;
-        s += v[j];
-    }
+    for (int j = 0; j < SCAN_ITEMS; j++) s += v[j];
     long long inc = tsg_wave_incl_scan(s);
     __shared__ long long ws[SCAN_THREADS / 64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -76,11 +90,20 @@ k_scan_down(const int *in, long long n, const long long *__restrict__ partial, O
     __syncthreads();
     long long off = partial[blockIdx.x] + inc - s;
     for (int k = 0; k < w; k++) off += ws[k];
+    int o[SCAN_ITEMS];
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
-        long long i = base + j;
-        if (i < n) out[i] = (OutT)off;
+        o[j] = (int)off;
         off += v[j];
+    }
+    if (vec) {
+        int4 *out4 = reinterpret_cast<int4 *>(out + base);
+#pragma unroll
+        for (int q = 0; q < 4; q++) out4[q] = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; j++)
+            if (base + j < n) out[base + j] = o[j];
     }
 }
 
@@ -102,7 +125,7 @@ int tsg_scan_i32_enqueue(hipStream_t stream, const int *in, int *out, long long 
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, partial, nb);
     TSG_LAUNCH_CHECK();
     if (n > 0) {
-        hipLaunchKernelGGL(k_scan_down<int>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, stream, in, n,
+        hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, stream, in, n,
                            (const long long *)partial, out);
         TSG_LAUNCH_CHECK();
     }
