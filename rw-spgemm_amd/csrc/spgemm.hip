@@ -378,6 +378,29 @@ __device__ __forceinline__ int lower_bound_i(const int *__restrict__ list, int l
     return lo;
 }
 
+// MODE 1 segments, one thread per segment (a hub tile-row has thousands, each
+// found by a binary search; one thread per row left the whole launch
+// waiting on the longest row): segment j's row is the last row whose first
+// segment is at or before j (off[] is the scanned per-row count, off[tilem]
+// the total), and its bounds are the same tiles seg_walk finds.
+__global__ void __launch_bounds__(256)
+k_seg1_fill(const int *__restrict__ Cptr, const int *__restrict__ Cnnz, int tilem, const int *__restrict__ off,
+            int max_nnz, int4 *__restrict__ items) {
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= off[tilem]) return;
+    int lo = 0, hi = tilem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    const int I = lo, jj = (int)(j - off[I]), n = off[I + 1] - off[I];
+    const int c0 = Cptr[I], u = Cptr[I + 1] - c0;
+    const int H = max_nnz - 255, base = Cnnz[c0];
+    const int s0 = jj == 0 ? 0 : lower_bound_i(Cnnz, c0, c0 + u - 1, base + jj * H) - c0;
+    const int s1 = jj + 1 == n ? u : lower_bound_i(Cnnz, c0, c0 + u - 1, base + (jj + 1) * H) - c0;
+    items[j] = make_int4(I, s0, s1, jj);
+}
+
 // ------------------------------------------------------------------ step 2
 // Row-wise (Gustavson) bitmask symbolic: one 256-thread block per segment
 // of a C tile-row.  Every (A tile (I,K), B tile (K,J)) pair of the row
@@ -2328,12 +2351,20 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
 // w.segcnt[tilem].
 template <int MODE>
 static int launch_segments(hipStream_t s, const tsg_dtiles *C, const SpgemmWork &w, int want, int max_tiles,
-                           int max_nnz, int4 *items, const unsigned char *cls) {
+                           int max_nnz, int4 *items, const unsigned char *cls, long long cap = 0) {
     const unsigned g = tsg_div_up(C->tilem, 256);
     hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem, cls, want,
                        max_tiles, max_nnz, w.segcnt, (int4 *)nullptr);
     TSG_LAUNCH_CHECK();
     TSG_TRY(tsg_scan_i32_enqueue(s, w.segcnt, w.segcnt, C->tilem, w.part3));
+    if (MODE == 1) {  // one thread per segment, cap = the segment array's capacity (>= the count)
+        if (cap > 0) {
+            hipLaunchKernelGGL(k_seg1_fill, dim3((unsigned)tsg_div_up(cap, 256)), dim3(256), 0, s, C->tile_ptr,
+                               C->tile_nnz, C->tilem, (const int *)w.segcnt, max_nnz, items);
+            TSG_LAUNCH_CHECK();
+        }
+        return TSG_OK;
+    }
     hipLaunchKernelGGL(k_seg<MODE>, dim3(g), dim3(256), 0, s, C->tile_ptr, C->tile_nnz, C->tilem, cls, want,
                        max_tiles, max_nnz, w.segcnt, items);
     TSG_LAUNCH_CHECK();
@@ -2399,7 +2430,7 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
         if (ev) TSG_HIP(hipEventRecord(ev[1], s));
     }
     if (!P.all && P.s3_unfit && P.s3_elem) {
-        TSG_TRY(launch_segments<1>(s, C, w, 0, 0x7fffffff, G3E_NNZ, w.seg3, w.rowcls));
+        TSG_TRY(launch_segments<1>(s, C, w, 0, 0x7fffffff, G3E_NNZ, w.seg3, w.rowcls, w.seg3_cap));
         hipLaunchKernelGGL(P.s3pk ? k_step3_elem<true> : k_step3_elem<false>, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
