@@ -15,9 +15,13 @@
 //   sorted-list intersection, 64-bit ballots to keep them in K order).
 // Step 3 (numeric, src/tilespgemm-cuda.h:1505-2608): k_step3_rows, one
 //   block per C tile-row, accumulates the row's C values in LDS in their
-//   final order, staging A tile K and B tile-row K per K-step (software
-//   pipelined) and running the (B tile, A row) tasks that meet nonempty B
-//   rows, compacted per wave.  Larger tile-rows use k_step3 (one C tile per
+//   final order; each C row belongs to one wave, which takes its tasks in
+//   ascending K.  A tile K with several entries has A tile K and B tile-row
+//   K staged into LDS (software pipelined) and the (B tile, A row) tasks
+//   that meet nonempty B rows run compacted per wave; a single-entry A tile
+//   is handled by its row's wave from registers.  Hypersparse and hub rows
+//   use k_step3_elem / k_step3_heavy (B's row index); larger tile-rows
+//   otherwise use k_step3 (one C tile per
 //   16-lane group, lane r owns tile row r, dense or compact LDS row
 //   accumulator).  No atomics anywhere: every C entry receives its
 //   products in ascending inner index k, the order of the serial SPA
@@ -1535,12 +1539,12 @@ __host__ __device__ inline size_t g3_layout(const G3Layout &L, size_t *off) {
 // Pipeline stage registers.  Staging is split by wave so every branch is
 // wave-uniform: wave 0 loads the B tile masks (uint4 = 8 rows, 2 per tile),
 // row offsets (uint4 per tile) and nnz offsets; wave 1 the A tile (values,
-// masks, row offsets); waves 2-3 the B values (4 per lane).  A stage is
-// loaded three K-steps before it is computed (register ring of 3).  The
-// roles share one set of 10 registers per stage: wave 0 keeps its two
-// 16-byte words as raw bits in d0:d1 and d2:d3 (moved, never computed on).
-// The ring runs over the staged (multi-entry) K-steps only: a stage is
-// loaded two staged steps before it is computed and stored one before.
+// masks, row offsets); waves 2-3 the B values (4 per lane).  The ring has
+// two register slots and runs over the staged (multi-entry) K-steps only: a
+// stage is loaded two staged steps before it is computed and stored into
+// LDS one staged step before.  The roles share one set of 10 registers per
+// slot: wave 0 keeps its two 16-byte words as raw bits in d0:d1 and d2:d3
+// (moved, never computed on).
 #define G3_SLOT_DECL(S)                                                   \
     double S##d0 = 0.0, S##d1 = 0.0, S##d2 = 0.0, S##d3 = 0.0;            \
     unsigned S##e0 = 0u, S##e1 = 0u
