@@ -1173,7 +1173,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
             double pv[16];
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                const int qs0 = __shfl(e0, gb + q, 64), qs1 = __shfl(e1, gb + q, 64);
+                const int qs0 = tsg_row16_bcast(e0, q), qs1 = tsg_row16_bcast(e1, q);
                 const int e = qs0 + l;
                 pc[q] = 0x7fffffff;
                 pv[q] = 0.0;
@@ -1184,11 +1184,11 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
             }
 #pragma unroll
             for (int q = 0; q < 16; q++) {  // sources in ascending k
-                const double as = __shfl(a, gb + q, 64);
-                const int qs0 = __shfl(e0, gb + q, 64), qs1 = __shfl(e1, gb + q, 64);
+                const double as = tsg_row16_bcast(a, q);
+                const int qs0 = tsg_row16_bcast(e0, q), qs1 = tsg_row16_bcast(e1, q);
                 if (pc[q] < col_hi) acc_add(pc[q], as, pv[q]);
                 // entries past the first 16 (the last prefetched one in range)
-                if (qs1 - qs0 > 16 && __shfl(pc[q], gb + 15, 64) < col_hi) {
+                if (qs1 - qs0 > 16 && tsg_row16_bcast(pc[q], 15) < col_hi) {
                     for (int e = qs0 + 16 + l; e < qs1; e += 16) {
                         const int col = Bcolx[e];
                         if (col >= col_hi) break;

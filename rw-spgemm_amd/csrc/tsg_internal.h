@@ -216,6 +216,25 @@ __device__ __forceinline__ T tsg_group16_incl_scan(T v) {
     }
     return v;
 }
+// Value of lane q of this lane's group of 16 lanes (q a constant after
+// unrolling): a DPP row broadcast, a VALU move with no LDS traffic, where
+// __shfl would issue a ds_bpermute.  Every lane of the group must be active.
+#define TSG_RB_CASE(Q) case Q: return __builtin_amdgcn_mov_dpp(v, 0x150 + Q, 0xF, 0xF, false);
+__device__ __forceinline__ int tsg_row16_bcast(int v, int q) {
+    switch (q) {
+        TSG_RB_CASE(0) TSG_RB_CASE(1) TSG_RB_CASE(2) TSG_RB_CASE(3)
+        TSG_RB_CASE(4) TSG_RB_CASE(5) TSG_RB_CASE(6) TSG_RB_CASE(7)
+        TSG_RB_CASE(8) TSG_RB_CASE(9) TSG_RB_CASE(10) TSG_RB_CASE(11)
+        TSG_RB_CASE(12) TSG_RB_CASE(13) TSG_RB_CASE(14) default: break;
+    }
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + 15, 0xF, 0xF, false);
+}
+#undef TSG_RB_CASE
+__device__ __forceinline__ double tsg_row16_bcast(double v, int q) {
+    const long long b = __double_as_longlong(v);
+    const int lo = tsg_row16_bcast((int)(unsigned)b, q), hi = tsg_row16_bcast((int)(b >> 32), q);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 template <typename T>
 __device__ __forceinline__ T tsg_group16_sum(T v) {
 #pragma unroll
