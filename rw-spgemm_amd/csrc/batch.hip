@@ -252,7 +252,7 @@ static int plan_group(tsg_batch *b, tsg_batch_group &g) {
         tsg_set_error("batch: group C nonzeros %lld above the cap %d", g.plan.nnzC, b->tune.batch_group_nnz_cap);
         return TSG_ERR_OVERFLOW;
     }
-    TSG_TRY(tsg_alloc(ctx, &g.work, tsg_spgemm_work_bytes(&g.C, g.plan.s1w)));
+    TSG_TRY(tsg_alloc(ctx, &g.work, tsg_spgemm_work_bytes(&g.C, &g.B, g.plan.s1w)));
     return TSG_OK;
 }
 

@@ -48,7 +48,7 @@ long long c_bytes(long long tilem, long long numtile, long long nnz, int s1w) {
     c.numtile = (int)numtile;
     c.nnz = (int)nnz;
     const long long arrays = 4 * (tilem + 1) + 12 * (numtile + 1) + 48 * numtile + 32 + 9 * (nnz + 1);
-    return arrays + (long long)tsg_spgemm_work_bytes(&c, s1w);
+    return arrays + (long long)tsg_spgemm_work_bytes(&c, nullptr, s1w);
 }
 
 // Device bytes a tile matrix allocated on ctx holds.

@@ -43,13 +43,78 @@ extern "C" void tsg_tune_default(tsg_tune *t) {
     t->s1_window_words = 0;
 }
 
+// ------------------------------------------------------------------ B tile-row bitmaps
+// For step 1: the tile columns of B tile-row K as a bitmap of nw 32-bit
+// words from absolute word F (bit b of word w <-> tile column 32 (F + w) + b),
+// kept at bits + off when it is at most about twice the size of the row's
+// column list (nw <= 2 * tiles + 2); desc[K] = {off, F, nw, 0} with nw = 0
+// for a row that is empty or not kept.  off = 2 * Bptr[K] + 2 * K bounds
+// every row's room without a scan, so this is one launch (graph-capturable,
+// every word written, no memset) and 8 * (numtile + tilem) bytes of bits.
+// any (may be NULL, zeroed beforehand) is set when some row is kept.
+#define BROW_CHUNK 2048  // words per LDS pass of a wave
+__global__ void __launch_bounds__(256)
+k_brow_bits(const int *__restrict__ Bptr, const int *__restrict__ Bcol, int tilem, int4 *__restrict__ desc,
+            unsigned *__restrict__ bits, int *__restrict__ any) {
+    __shared__ unsigned s_w[4][BROW_CHUNK];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int K0 = (blockIdx.x * 4 + wave) * 64;  // the wave's 64 rows, one per lane
+    if (K0 >= tilem) return;  // whole waves
+    unsigned *sw = s_w[wave];
+    const int K = K0 + lane;
+    int b0 = 0, n = 0, F = 0, nw = 0;
+    if (K < tilem) {
+        b0 = Bptr[K];
+        n = Bptr[K + 1] - b0;
+        if (n > 0) {
+            F = Bcol[b0] >> 5;
+            nw = (Bcol[b0 + n - 1] >> 5) - F + 1;
+            if (nw > 2 * n + 2) nw = 0;
+        }
+        desc[K] = make_int4(2 * b0 + 2 * K, F, nw, 0);
+    }
+    unsigned long long kept = __ballot(nw > 0);
+    // the same value from every writer; once it reads 1, no more stores
+    if (any && kept && lane == 0 && *(volatile int *)any == 0) *(volatile int *)any = 1;
+    while (kept) {  // each kept row by the whole wave
+        const int j = __ffsll((long long)kept) - 1;
+        kept &= kept - 1;
+        const int rb0 = __builtin_amdgcn_readlane(b0, j), rb1 = rb0 + __builtin_amdgcn_readlane(n, j);
+        const int rF = __builtin_amdgcn_readlane(F, j), rnw = __builtin_amdgcn_readlane(nw, j);
+        const int off = 2 * rb0 + 2 * (K0 + j);
+        int p = rb0;  // the row's next tile
+        for (int W = 0; W < rnw; W += BROW_CHUNK) {
+            const int cw = min(BROW_CHUNK, rnw - W);
+            for (int k = lane; k < cw; k += 64) sw[k] = 0u;
+            tsg_wave_sync();
+            const int lim = (rF + W + cw) << 5;  // first tile column past this pass
+            for (;;) {                           // the pass's tiles: a prefix of the rest
+                const int jj = p + lane;
+                const int c = jj < rb1 ? Bcol[jj] : 0x7fffffff;
+                if (c < lim) atomicOr(&sw[(c >> 5) - rF - W], 1u << (c & 31));
+                const int took = __popcll(__ballot(c < lim));
+                p += took;
+                if (took < 64) break;
+            }
+            tsg_wave_sync();
+            for (int k = lane; k < cw; k += 64) bits[(long long)off + W + k] = sw[k];
+            tsg_wave_sync();
+        }
+    }
+}
+
 // ------------------------------------------------------------------ step 1
+// Bdesc / Bbits (may be NULL): B tile-rows kept as bitmaps (k_brow_bits).
+// Their candidates are not enumerated: each lane ORs the bitmap words over
+// its own window words, with no search and no LDS atomics.  Bany (may be
+// NULL) is k_brow_bits' flag: when no row was kept, the descriptors are
+// not read at all.
 template <bool FILL, int BMW>
 __global__ void __launch_bounds__(64 * S1_WAVES)
 k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
-        const int *__restrict__ Bptr, const int *__restrict__ Bcol,
-        int *__restrict__ Cptr_or_cnt, int *__restrict__ Ccol, int *__restrict__ Crow, unsigned *__restrict__ bmstore,
-        int *__restrict__ rowspan) {
+        const int *__restrict__ Bptr, const int *__restrict__ Bcol, const int4 *__restrict__ Bdesc,
+        const unsigned *__restrict__ Bbits, const int *__restrict__ Bany, int *__restrict__ Cptr_or_cnt,
+        int *__restrict__ Ccol, int *__restrict__ Crow, unsigned *__restrict__ bmstore, int *__restrict__ rowspan) {
     __shared__ unsigned s_bm[S1_WAVES][BMW];
     __shared__ int s_pre[S1_WAVES][64];
     __shared__ int s_b0[S1_WAVES][64];
@@ -59,6 +124,7 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     unsigned *bm = s_bm[wave];
     int *pre = s_pre[wave], *pb0 = s_b0[wave];
     const int a0 = Aptr[I], a1 = Aptr[I + 1];
+    if (Bany && *Bany == 0) Bdesc = nullptr;  // wave-uniform
 
     // range of candidate tile columns: B rows are sorted
     int jmin = 0x7fffffff, jmax = -1;
@@ -72,12 +138,15 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     }
     jmin = tsg_wave_min(jmin);
     jmax = tsg_wave_max(jmax);
+    // windows start on a word boundary of absolute tile columns (the words
+    // of B's tile-row bitmaps)
+    const int jb = jmin & ~31;
 
     // a row spanning 2..64 windows (a periodic grid's boundary rows) first
     // finds the windows its candidates hit and visits only those
     constexpr int WSH = __builtin_ctz(32 * BMW);  // log2 of the columns per window
     unsigned long long wmask = ~0ull;
-    if (jmax >= jmin && ((jmax - jmin) >> WSH) >= 1 && ((jmax - jmin) >> WSH) < 64) {
+    if (jmax >= jmin && ((jmax - jb) >> WSH) >= 1 && ((jmax - jb) >> WSH) < 64) {
         unsigned wlo = 0u, whi = 0u;
         for (int c0 = a0; c0 < a1; c0 += 64) {
             const int p = c0 + lane;
@@ -86,6 +155,17 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
                 const int K = Acol[p];
                 b0 = Bptr[K];
                 len = Bptr[K + 1] - b0;
+                if (Bdesc) {
+                    const int4 d = Bdesc[K];
+                    if (d.z > 0) {  // a kept row marks the windows its words span
+                        len = 0;
+                        const int i0 = (max(d.y << 5, jb) - jb) >> WSH;
+                        const int i1 = (min(((d.y + d.z) << 5) - 1, jmax) - jb) >> WSH;
+                        const unsigned long long m = (i1 >= 63 ? ~0ull : (2ull << i1) - 1) & ~((1ull << i0) - 1);
+                        wlo |= (unsigned)m;
+                        whi |= (unsigned)(m >> 32);
+                    }
+                }
             }
             const int incl = tsg_wave_incl_scan(len);
             const int tot = __shfl(incl, 63, 64);
@@ -97,7 +177,7 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (pre[lo + st] <= c) lo += st;
-                const int wi = (Bcol[pb0[lo] + (c - pre[lo])] - jmin) >> WSH;
+                const int wi = (Bcol[pb0[lo] + (c - pre[lo])] - jb) >> WSH;
                 if (wi < 32) wlo |= 1u << wi; else whi |= 1u << (wi - 32);
             }
             tsg_wave_sync();
@@ -114,10 +194,12 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     const int base = FILL ? Cptr_or_cnt[I] : 0;
     // a row whose candidates fit one window keeps the count pass's bitmap
     // (bmstore, BMW words per tile-row) so the fill pass reads it back
-    const bool kept = bmstore != nullptr && jmax >= jmin && jmax - jmin < 32 * BMW;
+    const bool kept = bmstore != nullptr && jmax >= jmin && jmax - jb < 32 * BMW;
     unsigned *keep = kept ? bmstore + (long long)I * BMW : nullptr;
-    for (int w0 = jmin; w0 <= jmax; w0 += 32 * BMW) {
-        const int wi = (w0 - jmin) >> WSH;
+    constexpr int V = BMW / 64;            // window words per lane: lane + 64 v
+    constexpr int U = BMW == 256 ? 4 : 1;  // kept B tile-rows whose words are loaded together
+    for (int w0 = jb; w0 <= jmax; w0 += 32 * BMW) {
+        const int wi = (w0 - jb) >> WSH;
         if (wi < 64 && !((wmask >> wi) & 1ull)) continue;  // no candidate in this window
         const int nw = min(BMW, ((jmax - w0) >> 5) + 1);
         if (FILL && kept) {
@@ -126,13 +208,53 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
         } else {
             for (int k = lane; k < nw; k += 64) bm[k] = 0u;
             tsg_wave_sync();
+            const int W0 = w0 >> 5;  // absolute word of the window's first word
             for (int c0 = a0; c0 < a1; c0 += 64) {
                 const int p = c0 + lane;
                 int b0 = 0, len = 0;
+                int4 d = make_int4(0, 0, 0, 0);
                 if (p < a1) {
                     const int K = Acol[p];
                     b0 = Bptr[K];
                     len = Bptr[K + 1] - b0;
+                    if (Bdesc) {
+                        d = Bdesc[K];
+                        if (d.z > 0) len = 0;
+                    }
+                }
+                // kept B tile-rows: window words [lo, hi) of each, U rows' loads
+                // in flight; a lane ORs only its own words (lane + 64 v), so
+                // no atomics, and LDS order within the wave keeps these
+                // before the candidate atomics below
+                unsigned long long dm = __ballot(d.z > 0);
+                while (dm) {
+                    unsigned t[U][V];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        int lo = 0, hi = 0, base = 0;
+                        if (dm) {
+                            const int j = __ffsll((long long)dm) - 1;
+                            dm &= dm - 1;
+                            const int dF = __builtin_amdgcn_readlane(d.y, j);
+                            const int dn = __builtin_amdgcn_readlane(d.z, j);
+                            lo = max(W0, dF) - W0;
+                            hi = min(W0 + nw, dF + dn) - W0;
+                            base = __builtin_amdgcn_readlane(d.x, j) + W0 - dF;
+                        }
+#pragma unroll
+                        for (int v = 0; v < V; v++) {
+                            const int k = 64 * v + lane;
+                            t[u][v] = 0u;
+                            if (64 * v + 64 > lo && 64 * v < hi && k >= lo && k < hi) t[u][v] = Bbits[base + k];
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        unsigned x = 0u;
+#pragma unroll
+                        for (int u = 0; u < U; u++) x |= t[u][v];
+                        if (x) bm[64 * v + lane] |= x;
+                    }
                 }
                 const int incl = tsg_wave_incl_scan(len);
                 const int tot = __shfl(incl, 63, 64);
@@ -2112,7 +2234,8 @@ extern "C" int tsg_spgemm_count_tiles(tsg_ctx *ctx, const tsg_dtiles *A, const t
     TSG_TRY(tsg_alloc_t(ctx, &cnt, (size_t)A->tilem + 1));
     if (A->tilem > 0) {
         hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
-                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
+                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx,
+                           (const int4 *)nullptr, (const unsigned *)nullptr, (const int *)nullptr, cnt,
                            (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr);
         TSG_LAUNCH_CHECK();
     }
@@ -2210,7 +2333,8 @@ extern "C" int tsg_spgemm_count_tiles_noalloc(tsg_ctx *ctx, const tsg_dtiles *A,
     int nparts = (int)std::min<long long>(NOALLOC_PARTS, std::max(1u, tsg_div_up(A->tilem, 256)));
     if (A->tilem > 0) {
         hipLaunchKernelGGL((k_step1<false, 256>), dim3(tsg_div_up(A->tilem, S1_WAVES)), dim3(64 * S1_WAVES), 0,
-                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx, cnt,
+                           ctx->stream, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr, B->tile_colidx,
+                           (const int4 *)nullptr, (const unsigned *)nullptr, (const int *)nullptr, cnt,
                            (int *)nullptr, (int *)nullptr, (unsigned *)nullptr, (int *)nullptr);
         TSG_LAUNCH_CHECK();
     }
@@ -2255,23 +2379,50 @@ static size_t keep_bytes(long long tilem, int words) {
     return b <= ((size_t)1 << 30) ? b : 0;
 }
 
+// B's tile-row bitmaps for step 1 (k_brow_bits): a descriptor per B
+// tile-row, then the words.  0 when the word offsets would not fit 32 bits
+// (step 1 then enumerates every candidate).
+static size_t brow_bits_bytes(const tsg_dtiles *B) {
+    if (!B || B->tilem <= 0) return 0;
+    const long long words = 2ll * B->numtile + 2ll * B->tilem;
+    if (words >= 0x7fffffffLL) return 0;
+    return (size_t)B->tilem * 16 + (size_t)words * 4;
+}
+static int launch_brow_bits(hipStream_t s, const tsg_dtiles *B, void *buf, const int4 **desc,
+                            const unsigned **bits, int *any) {
+    *desc = nullptr;
+    *bits = nullptr;
+    if (!buf || !brow_bits_bytes(B)) return TSG_OK;
+    int4 *d = (int4 *)buf;
+    unsigned *w = (unsigned *)((char *)buf + (size_t)B->tilem * 16);
+    hipLaunchKernelGGL(k_brow_bits, dim3(tsg_div_up(B->tilem, 256)), dim3(256), 0, s, (const int *)B->tile_ptr,
+                       (const int *)B->tile_colidx, B->tilem, d, w, any);
+    TSG_LAUNCH_CHECK();
+    *desc = d;
+    *bits = w;
+    return TSG_OK;
+}
+
 static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C, bool fill,
-                        int words, unsigned *bmstore, int *rowspan = nullptr) {
+                        int words, unsigned *bmstore, const int4 *bdesc, const unsigned *bbits,
+                        const int *bany, int *rowspan = nullptr) {
     if (C->tilem <= 0) return TSG_OK;
     const dim3 g(tsg_div_up(C->tilem, S1_WAVES)), blk(64 * S1_WAVES);
     const bool wide = words == 2048;
     if (!fill && !wide)
         hipLaunchKernelGGL((k_step1<false, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
+                           B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
     else if (!fill)
         hipLaunchKernelGGL((k_step1<false, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
+                           B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
     else if (!wide)
         hipLaunchKernelGGL((k_step1<true, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore, (int *)nullptr);
+                           B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore,
+                           (int *)nullptr);
     else
         hipLaunchKernelGGL((k_step1<true, 2048>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
-                           B->tile_colidx, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore, (int *)nullptr);
+                           B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, C->tile_colidx, C->tile_rowidx, bmstore,
+                           (int *)nullptr);
     TSG_LAUNCH_CHECK();
     return TSG_OK;
 }
@@ -2285,6 +2436,7 @@ struct SpgemmWork {
     unsigned char *rowcls;
     int *cls_out;
     int *cls_part;  // k_row_class partials, 8 ints per block
+    void *bbits;       // B's tile-row bitmaps for step 1 (may be NULL)
     int4 *seg2;        // step-2 segments
     unsigned *bmstore; // step-1 bitmaps kept from the count pass (may be NULL)
     int4 *seg3;        // step-3 element segments
@@ -2304,16 +2456,17 @@ static size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
 // Offsets of the work arrays inside one buffer; returns its size.  The
 // step-3 segments go last (the synchronous pipeline adds them only once
 // C's nonzero count is known).
-#define WORK_NARR 10
+#define WORK_NARR 11
 
 static size_t work_layout(long long tilem, long long numtile, long long nnz, size_t off[WORK_NARR],
-                          int s1w = 256) {
+                          int s1w = 256, size_t bbytes = 0) {
     const size_t sz[WORK_NARR] = {
         8 * (size_t)tsg_scan_partial_count(tilem), 8 * (size_t)tsg_scan_partial_count(numtile),
         8 * (size_t)tsg_scan_partial_count(tilem), 4 * ((size_t)tilem + 1), (size_t)tilem + 1, 32,
         16 * (size_t)seg_bound(tilem, numtile, G2_UMAX, 0, 0),
         keep_bytes(tilem, s1w),  // step-1 bitmaps (enqueue-only pipeline)
         32 * (size_t)((tilem + 15) / 16 + 1),  // row-class partials, 8 ints per block
+        bbytes,                                // B's tile-row bitmaps (enqueue-only pipeline)
         16 * (size_t)(tilem + nnz / (G3E_NNZ - 255) + 2)};
     size_t o = 0;
     for (int i = 0; i < WORK_NARR; i++) {
@@ -2324,9 +2477,9 @@ static size_t work_layout(long long tilem, long long numtile, long long nnz, siz
 }
 
 static SpgemmWork work_at(void *base, long long tilem, long long numtile, long long nnz, void *seg3_base,
-                          int s1w = 256) {
+                          int s1w = 256, size_t bbytes = 0) {
     size_t off[WORK_NARR];
-    work_layout(tilem, numtile, nnz, off, s1w);
+    work_layout(tilem, numtile, nnz, off, s1w, bbytes);
     char *b = (char *)base;
     SpgemmWork w;
     w.part1 = (long long *)(b + off[0]);
@@ -2338,7 +2491,8 @@ static SpgemmWork work_at(void *base, long long tilem, long long numtile, long l
     w.seg2 = (int4 *)(b + off[6]);
     w.bmstore = keep_bytes(tilem, s1w) ? (unsigned *)(b + off[7]) : nullptr;
     w.cls_part = (int *)(b + off[8]);
-    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[9]);
+    w.bbits = bbytes ? (void *)(b + off[9]) : nullptr;
+    w.seg3 = (int4 *)(seg3_base ? (char *)seg3_base : b + off[10]);
     w.seg2_cap = seg_bound(tilem, numtile, G2_UMAX, 0, 0);
     w.seg3_cap = tilem + nnz / (G3E_NNZ - 255) + 2;
     w.hflag = nullptr;
@@ -2471,7 +2625,7 @@ static int alloc_c_tiles(tsg_ctx *ctx, tsg_dtiles *C, long long numtileC) {
 // the context stream).
 struct SpgemmTemps {
     unsigned *keep = nullptr;
-    void *scr = nullptr, *wbuf = nullptr, *w3buf = nullptr;
+    void *scr = nullptr, *wbuf = nullptr, *w3buf = nullptr, *bb = nullptr;
     unsigned char *hflag = nullptr;
     int *hcnt = nullptr, *hinfo = nullptr, *hcur = nullptr, *span = nullptr;
     int2 *hitems = nullptr;
@@ -2480,7 +2634,7 @@ struct SpgemmTemps {
 };
 
 static void free_temps(tsg_ctx *ctx, SpgemmTemps &t) {
-    void *ps[] = {t.keep, t.scr, t.wbuf, t.w3buf, t.hflag, t.hcnt, t.hinfo, t.hcur, t.hitems, t.span};
+    void *ps[] = {t.keep, t.scr, t.wbuf, t.w3buf, t.hflag, t.hcnt, t.hinfo, t.hcur, t.hitems, t.span, t.bb};
     for (void *p : ps) tsg_free(ctx, p);
     for (int i = 0; i < t.nev; i++) (void)hipEventDestroy(t.ev[i]);
     t = SpgemmTemps();
@@ -2539,9 +2693,16 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     void *scr = T.scr;
     unsigned long long *bins = (unsigned long long *)scr;
     int *st2 = reinterpret_cast<int *>(static_cast<char *>(scr) + 32);  // step-2 sizing maxima
+    int *bany = reinterpret_cast<int *>(static_cast<char *>(scr) + 48);  // some B tile-row bitmap kept
     TSG_HIP(hipMemsetAsync(scr, 0, 64, s));
     TSG_TRY(tsg_alloc_t(ctx, &T.span, (size_t)C->tilem + 1));
-    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep, T.span));
+    const int4 *bdesc = nullptr;
+    const unsigned *bbits = nullptr;
+    if (const size_t bbb = C->tilem > 0 ? brow_bits_bytes(B) : 0) {
+        TSG_TRY(tsg_alloc(ctx, &T.bb, bbb));
+        TSG_TRY(launch_brow_bits(s, B, T.bb, &bdesc, &bbits, bany));
+    }
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, T.keep, bdesc, bbits, bany, T.span));
     if (C->tilem > 0) {
         hipLaunchKernelGGL(k_row_stats, dim3(std::min<unsigned>(ROW_STATS_BLOCKS, tsg_div_up(C->tilem, 1024))),
                            dim3(1024), 0, s, (const int *)C->tile_ptr, (const int *)T.span, C->tilem, st2);
@@ -2579,9 +2740,11 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     (void)wbytes;
     void *wbuf = T.wbuf;
     SpgemmWork w = work_at(wbuf, C->tilem, numtileC, 0, nullptr);
-    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, T.keep));
+    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, T.keep, bdesc, bbits, bany));
     tsg_free(ctx, T.keep);
     T.keep = nullptr;
+    tsg_free(ctx, T.bb);
+    T.bb = nullptr;
     if (timed) TSG_HIP(hipEventRecord(ev[1], s));
     // step-2 LDS sizing: the largest segment and the widest tile-row span
     P.s2u = G2_UMAX;
@@ -2734,17 +2897,20 @@ extern "C" int tsg_spgemm(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B
     return tsg_spgemm_planned(ctx, A, B, tune_in, C, stats, nullptr);
 }
 
-size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, int s1w) {
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, const tsg_dtiles *B, int s1w) {
     size_t off[WORK_NARR];
-    return work_layout(C->tilem, C->numtile, C->nnz, off, s1w);
+    return work_layout(C->tilem, C->numtile, C->nnz, off, s1w, brow_bits_bytes(B));
 }
 
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
                        const tsg_spgemm_plan &P, void *work) {
-    const SpgemmWork w = work_at(work, C->tilem, C->numtile, C->nnz, nullptr, P.s1w);
-    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, w.bmstore));
+    const SpgemmWork w = work_at(work, C->tilem, C->numtile, C->nnz, nullptr, P.s1w, brow_bits_bytes(B));
+    const int4 *bdesc = nullptr;
+    const unsigned *bbits = nullptr;
+    if (C->tilem > 0) TSG_TRY(launch_brow_bits(s, B, w.bbits, &bdesc, &bbits, nullptr));
+    TSG_TRY(launch_step1(s, A, B, C, false, P.s1w, w.bmstore, bdesc, bbits, nullptr));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_ptr, C->tile_ptr, C->tilem, w.part1));
-    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, w.bmstore));
+    TSG_TRY(launch_step1(s, A, B, C, true, P.s1w, w.bmstore, bdesc, bbits, nullptr));
     const TileOperands op = make_operands(A, B);
     TSG_TRY(launch_step2(s, op, B, C, P, w));
     TSG_TRY(tsg_scan_i32_enqueue(s, C->tile_nnz, C->tile_nnz, C->numtile, w.part2));

@@ -147,8 +147,9 @@ int tsg_spgemm_tile_rows_ex(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles 
                             const tsg_tune *tune, tsg_dtiles *C, tsg_stats *stats, bool symbolic);
 // Intermediate products of each A tile-row (device u64[A->tilem]).
 int tsg_tile_row_products(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, unsigned long long *d_out);
-// Device scratch bytes tsg_spgemm_enqueue needs for this C.
-size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, int s1w);
+// Device scratch bytes tsg_spgemm_enqueue needs for this C and B (B may be
+// NULL: no room for B's tile-row bitmaps, counted for a product with B).
+size_t tsg_spgemm_work_bytes(const tsg_dtiles *C, const tsg_dtiles *B, int s1w);
 // Step-1 bitmap window (32-bit words per wave) for products with this B.
 int tsg_s1_words(const tsg_dtiles *B, const tsg_tune &t);
 int tsg_spgemm_enqueue(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B, tsg_dtiles *C,
