@@ -1460,6 +1460,13 @@ k_heavy_list(const int *__restrict__ off, const unsigned char *__restrict__ flag
 #ifndef HEAVY_STAGE
 #define HEAVY_STAGE 8  // B entries per source staged in LDS per part
 #endif
+// lane q's double, read by the whole wave (q wave-uniform)
+__device__ __forceinline__ double rl_f64(double v, int q) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, q);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), q);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 __global__ void __launch_bounds__(64)
 k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, const double *__restrict__ Avalx,
@@ -1471,8 +1478,6 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
     __shared__ double s_acc[HEAVY_PART_COLS];
     __shared__ double s_sv[64 * HEAVY_STAGE];  // staged B values, source-major
     __shared__ int s_sc[64 * HEAVY_STAGE];     // their columns
-    __shared__ double s_a[64];
-    __shared__ int s_n[64];  // staged entries in the part | more may follow << 8
     const int lane = threadIdx.x;
     int *cur = cursor + (long long)blockIdx.x * cur_stride;
     const int ni = *nitems;
@@ -1498,9 +1503,10 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                 const int n = min(64, xe - xc);
                 int cb = 0, ce = 0, cnt = 0;
                 bool more = false;
+                double aq = 0.0;  // lane q: a_ik of source q
                 if (lane < n) {
                     const int k = Acolx[xc + lane];
-                    s_a[lane] = Avalx[xc + lane];
+                    aq = Avalx[xc + lane];
                     ce = Browp[k + 1];
                     cb = part == p0 ? lower_bound_i(Bcolx, Browp[k], ce, lo) : cur[xc - xa + lane];
                     int col[HEAVY_STAGE];
@@ -1520,8 +1526,10 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                         }
                     }
                     more = cnt == HEAVY_STAGE && cb + HEAVY_STAGE < ce;
-                    s_n[lane] = cnt | (more ? 256 : 0);
                 }
+                // lane q: staged entries of source q | more may follow << 8;
+                // read by the whole wave with v_readlane, not through LDS
+                const int nq = cnt | (more ? 256 : 0);
                 int next = cb + cnt;  // lane q: where source q's next part starts
                 // only sources with entries in the part take a step
                 unsigned long long act = __ballot(lane < n && cnt > 0);
@@ -1529,8 +1537,8 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                 // the staged data of the next active source is read before this
                 // source's accumulation, which then costs one LDS round trip
                 int q = act ? __ffsll((long long)act) - 1 : 0;
-                int sn = act ? s_n[q] : 0, sc = 0;
-                double a = act ? s_a[q] : 0.0, sv = 0.0;
+                int sn = act ? __builtin_amdgcn_readlane(nq, q) : 0, sc = 0;
+                double a = act ? rl_f64(aq, q) : 0.0, sv = 0.0;
                 if (lane < (sn & 255)) {
                     sc = s_sc[q * HEAVY_STAGE + lane];
                     sv = s_sv[q * HEAVY_STAGE + lane];
@@ -1541,8 +1549,8 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                     int sn1 = 0, sc1 = 0;
                     double a1 = 0.0, sv1 = 0.0;
                     if (act) {
-                        sn1 = s_n[q1];
-                        a1 = s_a[q1];
+                        sn1 = __builtin_amdgcn_readlane(nq, q1);
+                        a1 = rl_f64(aq, q1);
                         if (lane < (sn1 & 255)) {
                             sc1 = s_sc[q1 * HEAVY_STAGE + lane];
                             sv1 = s_sv[q1 * HEAVY_STAGE + lane];
@@ -1550,8 +1558,8 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                     }
                     if (lane < (sn & 255)) s_acc[sc - lo] = __builtin_fma(a, sv, s_acc[sc - lo]);
                     if (sn & 256) {  // entries past the staged ones, 256 per round trip
-                        int e = __shfl(cb, q, 64) + HEAVY_STAGE;
-                        const int e1 = __shfl(ce, q, 64);
+                        int e = __builtin_amdgcn_readlane(cb, q) + HEAVY_STAGE;
+                        const int e1 = __builtin_amdgcn_readlane(ce, q);
                         while (e < e1) {
                             int c[4];
                             double v[4];
