@@ -418,7 +418,7 @@ def rmat_cpu_baseline(a, b, target_s=10.0):
                       f"({tmin:.2f} s)"}
 
 
-def rmat_leg(ts, scale=20, chunk_products=1500000000, cpu=True):
+def rmat_leg(ts, scale=20, chunk_products=1500000000, cpu=True, orders=("random", "rcm", "degree")):
     """R-MAT scale-20 C = A*A^T (edge factor 16, (0.57, 0.19, 0.19, 0.05),
     symmetrized, duplicates merged, values U(0,1], seed 1) under the random,
     RCM and degree orderings, each computed on the device.  C (~2.5e10
@@ -433,7 +433,9 @@ def rmat_leg(ts, scale=20, chunk_products=1500000000, cpu=True):
     d = ctx.upload(a)
     out = {"scale": scale, "n": a.m, "nnz": a.nnz, "orders": {}}
     nnzc = set()
-    for algo in ("random", "rcm", "degree"):
+    for algo in orders:
+        if algo not in ("random", "rcm", "degree"):
+            raise ValueError(f"unknown R-MAT ordering {algo!r}")
         ctx.sync()
         t0 = time.perf_counter()
         if algo == "rcm":
@@ -505,6 +507,8 @@ def main():
     ap.add_argument("--rmat-only", action="store_true")
     ap.add_argument("--no-rmat", action="store_true")
     ap.add_argument("--rmat-scale", type=int, default=20)
+    ap.add_argument("--rmat-orders", default="random,rcm,degree",
+                    help="orderings of the --rmat-only leg, comma-separated")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rmat-scale", type=int, default=18)
@@ -520,7 +524,8 @@ def main():
     import tilespgemm as ts
 
     if args.rmat_only:
-        print(json.dumps({"rmat": rmat_leg(ts, scale=args.rmat_scale, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"rmat": rmat_leg(ts, scale=args.rmat_scale, cpu=not args.no_cpu,
+                                           orders=tuple(args.rmat_orders.split(",")))}), flush=True)
         return
     if args.serve_only:
         print(json.dumps({"served": serve_leg(ts, n_req=args.serve_requests, rmat_scale=args.serve_rmat_scale)}),

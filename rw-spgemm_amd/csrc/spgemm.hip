@@ -49,23 +49,18 @@ extern "C" void tsg_tune_default(tsg_tune *t) {
 // kept at bits + off when it is at most about twice the size of the row's
 // column list (nw <= 2 * tiles + 2); desc[K] = {off, F, nw, 0} with nw = 0
 // for a row that is empty or not kept.  off = 2 * Bptr[K] + 2 * K bounds
-// every row's room without a scan, so this is one launch (graph-capturable,
-// every word written, no memset) and 8 * (numtile + tilem) bytes of bits.
-// any (may be NULL, zeroed beforehand) is set when some row is kept.
-#define BROW_CHUNK 2048  // words per LDS pass of a wave
+// every row's room without a scan.  Two launches (graph-capturable, every
+// word of a kept row written once, no memset): k_brow_desc, one lane per
+// tile-row, then k_brow_fill, one lane per B tile, so a hub tile-row's
+// words are spread over as many waves as it has tiles.
 __global__ void __launch_bounds__(256)
-k_brow_bits(const int *__restrict__ Bptr, const int *__restrict__ Bcol, int tilem, int4 *__restrict__ desc,
-            unsigned *__restrict__ bits, int *__restrict__ any) {
-    __shared__ unsigned s_w[4][BROW_CHUNK];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int K0 = (blockIdx.x * 4 + wave) * 64;  // the wave's 64 rows, one per lane
-    if (K0 >= tilem) return;  // whole waves
-    unsigned *sw = s_w[wave];
-    const int K = K0 + lane;
-    int b0 = 0, n = 0, F = 0, nw = 0;
+k_brow_desc(const int *__restrict__ Bptr, const int *__restrict__ Bcol, int tilem, int4 *__restrict__ desc,
+            int *__restrict__ any) {
+    const int K = blockIdx.x * 256 + threadIdx.x;
+    int nw = 0;
     if (K < tilem) {
-        b0 = Bptr[K];
-        n = Bptr[K + 1] - b0;
+        const int b0 = Bptr[K], n = Bptr[K + 1] - b0;
+        int F = 0;
         if (n > 0) {
             F = Bcol[b0] >> 5;
             nw = (Bcol[b0 + n - 1] >> 5) - F + 1;
@@ -73,42 +68,74 @@ k_brow_bits(const int *__restrict__ Bptr, const int *__restrict__ Bcol, int tile
         }
         desc[K] = make_int4(2 * b0 + 2 * K, F, nw, 0);
     }
-    unsigned long long kept = __ballot(nw > 0);
-    // the same value from every writer; once it reads 1, no more stores
-    if (any && kept && lane == 0 && *(volatile int *)any == 0) *(volatile int *)any = 1;
-    while (kept) {  // each kept row by the whole wave
-        const int j = __ffsll((long long)kept) - 1;
-        kept &= kept - 1;
-        const int rb0 = __builtin_amdgcn_readlane(b0, j), rb1 = rb0 + __builtin_amdgcn_readlane(n, j);
-        const int rF = __builtin_amdgcn_readlane(F, j), rnw = __builtin_amdgcn_readlane(nw, j);
-        const int off = 2 * rb0 + 2 * (K0 + j);
-        int p = rb0;  // the row's next tile
-        for (int W = 0; W < rnw; W += BROW_CHUNK) {
-            const int cw = min(BROW_CHUNK, rnw - W);
-            for (int k = lane; k < cw; k += 64) sw[k] = 0u;
-            tsg_wave_sync();
-            const int lim = (rF + W + cw) << 5;  // first tile column past this pass
-            for (;;) {                           // the pass's tiles: a prefix of the rest
-                const int jj = p + lane;
-                const int c = jj < rb1 ? Bcol[jj] : 0x7fffffff;
-                if (c < lim) atomicOr(&sw[(c >> 5) - rF - W], 1u << (c & 31));
-                const int took = __popcll(__ballot(c < lim));
-                p += took;
-                if (took < 64) break;
-            }
-            tsg_wave_sync();
-            for (int k = lane; k < cw; k += 64) bits[(long long)off + W + k] = sw[k];
-            tsg_wave_sync();
+    // any (may be NULL, zeroed beforehand): set when some row is kept; the
+    // same value from every writer, and no store once it reads 1
+    if (any && __ballot(nw > 0) && (threadIdx.x & 63) == 0 && *(volatile int *)any == 0) *(volatile int *)any = 1;
+}
+
+// A run is the tiles of one kept row in one bitmap word (consecutive in the
+// row-major tile order).  Its last tile's lane writes the word (the run's
+// bits OR-ed across lanes, plus the run's tiles before the wave's first,
+// at most 31) and zeroes the empty words up to the row's next run.
+__global__ void __launch_bounds__(256)
+k_brow_fill(const int *__restrict__ Brow, const int *__restrict__ Bcol, long long numtile,
+            const int4 *__restrict__ desc, unsigned *__restrict__ bits) {
+    const int lane = threadIdx.x & 63;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long t0 = t - lane;  // the wave's first tile
+    if (t0 >= numtile) return;      // whole waves
+    const bool valid = t < numtile;
+    int K = -1, w = -1, col = 0;
+    int4 d = make_int4(0, 0, 0, 0);
+    if (valid) {
+        K = Brow[t];
+        d = desc[K];
+        col = Bcol[t];
+        if (d.z > 0) w = (col >> 5) - d.y;
+    }
+    const bool kept = valid && d.z > 0;
+    // the next tile's row and word (lane 63: loaded)
+    int Kn = __shfl_down(K, 1, 64), wn = __shfl_down(w, 1, 64), cn = __shfl_down(col, 1, 64);
+    if (lane == 63) {
+        Kn = -1;
+        if (t + 1 < numtile) {
+            Kn = Brow[t + 1];
+            cn = Bcol[t + 1];
+            wn = Kn == K ? (cn >> 5) - d.y : -1;
         }
+    }
+    const bool last = kept && !(Kn == K && wn == w);
+    // inclusive OR over the run's lanes up to this one
+    unsigned v = kept ? 1u << (col & 31) : 0u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned vv = __shfl_up(v, o, 64);
+        const int kk = __shfl_up(K, o, 64), ww = __shfl_up(w, o, 64);
+        if (lane >= o && kk == K && ww == w) v |= vv;
+    }
+    // lane 0's run, read with every lane active (a shuffle inside the branch
+    // below would not see lane 0 unless it is a run's last lane too)
+    const int K0 = __shfl(K, 0, 64), w0 = __shfl(w, 0, 64);
+    if (last) {
+        if (K0 == K && w0 == w)  // the run began before this wave
+            for (long long u = t0 - 1; u >= 0 && Brow[u] == K; u--) {
+                const int c = Bcol[u];
+                if ((c >> 5) - d.y != w) break;
+                v |= 1u << (c & 31);
+            }
+        unsigned *row = bits + d.x;
+        row[w] = v;
+        const int wend = Kn == K ? wn : d.z;  // the row's next run, or its end
+        for (int g = w + 1; g < wend; g++) row[g] = 0u;
     }
 }
 
 // ------------------------------------------------------------------ step 1
-// Bdesc / Bbits (may be NULL): B tile-rows kept as bitmaps (k_brow_bits).
-// Their candidates are not enumerated: each lane ORs the bitmap words over
-// its own window words, with no search and no LDS atomics.  Bany (may be
-// NULL) is k_brow_bits' flag: when no row was kept, the descriptors are
-// not read at all.
+// Bdesc / Bbits (may be NULL): B tile-rows kept as bitmaps (k_brow_desc,
+// k_brow_fill).  Their candidates are not enumerated: each lane ORs the
+// bitmap words over its own window words, with no search and no LDS
+// atomics.  Bany (may be NULL) is k_brow_desc's flag: when no row was kept,
+// the descriptors are not read at all.
 template <bool FILL, int BMW>
 __global__ void __launch_bounds__(64 * S1_WAVES)
 k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
@@ -1535,7 +1562,28 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                 unsigned long long act = __ballot(lane < n && cnt > 0);
                 tsg_wave_sync();
                 // the staged data of the next active source is read before this
-                // source's accumulation, which then costs one LDS round trip
+                // source's accumulation, which then costs one LDS round trip;
+                // so are the first 256 entries past a source's staged ones
+                // (global loads into registers, one source ahead)
+                auto tail_first = [&](int qq, int snq, int *tc, double *tv) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        tc[u] = 0x7fffffff;
+                        tv[u] = 0.0;
+                    }
+                    if (snq & 256) {
+                        const int e = __builtin_amdgcn_readlane(cb, qq) + HEAVY_STAGE;
+                        const int e1 = __builtin_amdgcn_readlane(ce, qq);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int ee = e + 64 * u + lane;
+                            if (ee < e1) {
+                                tc[u] = Bcolx[ee];
+                                tv[u] = Bvalx[ee];
+                            }
+                        }
+                    }
+                };
                 int q = act ? __ffsll((long long)act) - 1 : 0;
                 int sn = act ? __builtin_amdgcn_readlane(nq, q) : 0, sc = 0;
                 double a = act ? rl_f64(aq, q) : 0.0, sv = 0.0;
@@ -1543,6 +1591,9 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                     sc = s_sc[q * HEAVY_STAGE + lane];
                     sv = s_sv[q * HEAVY_STAGE + lane];
                 }
+                int tc[4];
+                double tv[4];
+                tail_first(q, sn, tc, tv);
                 while (act) {  // ascending k; entries of one k have distinct columns
                     act &= act - 1;
                     const int q1 = act ? __ffsll((long long)act) - 1 : 0;
@@ -1556,11 +1607,26 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                             sv1 = s_sv[q1 * HEAVY_STAGE + lane];
                         }
                     }
+                    int tc1[4];
+                    double tv1[4];
+                    tail_first(q1, sn1, tc1, tv1);
                     if (lane < (sn & 255)) s_acc[sc - lo] = __builtin_fma(a, sv, s_acc[sc - lo]);
                     if (sn & 256) {  // entries past the staged ones, 256 per round trip
                         int e = __builtin_amdgcn_readlane(cb, q) + HEAVY_STAGE;
                         const int e1 = __builtin_amdgcn_readlane(ce, q);
-                        while (e < e1) {
+                        bool go;
+                        {  // the first round, loaded one source ahead
+                            int got = 0;
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const bool in2 = tc[u] < hi;
+                                if (in2) s_acc[tc[u] - lo] = __builtin_fma(a, tv[u], s_acc[tc[u] - lo]);
+                                got += __popcll(__ballot(in2));
+                            }
+                            e += got;
+                            go = got == 256;  // the part may hold more of this row
+                        }
+                        while (go && e < e1) {
                             int c[4];
                             double v[4];
 #pragma unroll
@@ -1587,6 +1653,11 @@ k_step3_heavy(const int *__restrict__ Arowp, const int *__restrict__ Acolx, cons
                     sc = sc1;
                     a = a1;
                     sv = sv1;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        tc[u] = tc1[u];
+                        tv[u] = tv1[u];
+                    }
                 }
                 if (part + 1 < p1 && lane < n) cur[xc - xa + lane] = next;
             }
@@ -2387,11 +2458,12 @@ static size_t keep_bytes(long long tilem, int words) {
     return b <= ((size_t)1 << 30) ? b : 0;
 }
 
-// B's tile-row bitmaps for step 1 (k_brow_bits): a descriptor per B
-// tile-row, then the words.  0 when the word offsets would not fit 32 bits
-// (step 1 then enumerates every candidate).
+// B's tile-row bitmaps for step 1 (k_brow_desc, k_brow_fill): a descriptor
+// per B tile-row, then the words.  0 when B has no tile row index or the
+// word offsets would not fit 32 bits (step 1 then enumerates every
+// candidate).
 static size_t brow_bits_bytes(const tsg_dtiles *B) {
-    if (!B || B->tilem <= 0) return 0;
+    if (!B || B->tilem <= 0 || !B->tile_rowidx) return 0;
     const long long words = 2ll * B->numtile + 2ll * B->tilem;
     if (words >= 0x7fffffffLL) return 0;
     return (size_t)B->tilem * 16 + (size_t)words * 4;
@@ -2403,8 +2475,12 @@ static int launch_brow_bits(hipStream_t s, const tsg_dtiles *B, void *buf, const
     if (!buf || !brow_bits_bytes(B)) return TSG_OK;
     int4 *d = (int4 *)buf;
     unsigned *w = (unsigned *)((char *)buf + (size_t)B->tilem * 16);
-    hipLaunchKernelGGL(k_brow_bits, dim3(tsg_div_up(B->tilem, 256)), dim3(256), 0, s, (const int *)B->tile_ptr,
-                       (const int *)B->tile_colidx, B->tilem, d, w, any);
+    hipLaunchKernelGGL(k_brow_desc, dim3(tsg_div_up(B->tilem, 256)), dim3(256), 0, s, (const int *)B->tile_ptr,
+                       (const int *)B->tile_colidx, B->tilem, d, any);
+    if (B->numtile > 0)
+        hipLaunchKernelGGL(k_brow_fill, dim3(tsg_div_up(B->numtile, 256)), dim3(256), 0, s,
+                           (const int *)B->tile_rowidx, (const int *)B->tile_colidx, (long long)B->numtile,
+                           (const int4 *)d, w);
     TSG_LAUNCH_CHECK();
     *desc = d;
     *bits = w;

@@ -117,6 +117,34 @@ def test_row_kernel_single_entry_tiles(ctx, case):
     spgemm_check(ctx, a, t(a), label=f"single-entry tiles: {case}, AA^T")
 
 
+def test_step1_kept_bitmaps_runs_cross_waves(ctx):
+    """C = I * B with B's tile-rows holding runs of consecutive tile columns
+    (kept as step-1 bitmaps) of lengths that put the 64-tile boundaries of
+    the bitmap builder inside runs at every lane offset, next to sparse
+    tile-rows that are not kept: C's tile list is B's, compared with the
+    oracle array by array."""
+    rng = np.random.default_rng(17)
+    ntc = 400
+    n = 16 * ntc
+    rows, cols = [], []
+    for K in range(n // 16):
+        if K % 5 == 4:  # sparse: a few tiles far apart (not kept)
+            js = np.unique(rng.integers(0, ntc, 3))
+        else:  # a run of consecutive tile columns, some gaps
+            L = 1 + (K * 37) % 150
+            j0 = int(rng.integers(0, ntc - L + 1))
+            js = np.arange(j0, j0 + L)
+            js = js[rng.random(L) < 0.9] if L > 4 else js
+        for J in js:
+            rows.append(16 * K + int(rng.integers(0, 16)))
+            cols.append(16 * int(J) + int(rng.integers(0, 16)))
+    rows, cols = np.array(rows), np.array(cols)
+    b = _coo(n, rows, cols, rng)
+    eye = _coo(n, np.arange(n), np.arange(n), rng)
+    st = spgemm_check(ctx, eye, b, label="I * B, kept bitmap runs")
+    assert st.numtileC == len(set(zip((rows // 16).tolist(), (cols // 16).tolist())))
+
+
 def test_wide_tilerow_uses_tile_parallel_step2(ctx):
     """A hub row touching 700 tile columns makes C tile-row 0 hold 700 tiles
     (> the 512 the row-wise step-2 kernel keeps in LDS), so that row goes
