@@ -79,11 +79,12 @@ k_brow_desc(const int *__restrict__ Bptr, const int *__restrict__ Bcol, int tile
 // at most 31) and zeroes the empty words up to the row's next run.
 __global__ void __launch_bounds__(256)
 k_brow_fill(const int *__restrict__ Brow, const int *__restrict__ Bcol, long long numtile,
-            const int4 *__restrict__ desc, unsigned *__restrict__ bits) {
+            const int4 *__restrict__ desc, unsigned *__restrict__ bits, const int *__restrict__ any) {
     const int lane = threadIdx.x & 63;
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long t0 = t - lane;  // the wave's first tile
     if (t0 >= numtile) return;      // whole waves
+    if (any && *any == 0) return;   // no row kept (k_brow_desc's flag, when given)
     const bool valid = t < numtile;
     int K = -1, w = -1, col = 0;
     int4 d = make_int4(0, 0, 0, 0);
@@ -2480,7 +2481,7 @@ static int launch_brow_bits(hipStream_t s, const tsg_dtiles *B, void *buf, const
     if (B->numtile > 0)
         hipLaunchKernelGGL(k_brow_fill, dim3(tsg_div_up(B->numtile, 256)), dim3(256), 0, s,
                            (const int *)B->tile_rowidx, (const int *)B->tile_colidx, (long long)B->numtile,
-                           (const int4 *)d, w);
+                           (const int4 *)d, w, (const int *)any);
     TSG_LAUNCH_CHECK();
     *desc = d;
     *bits = w;
