@@ -1428,7 +1428,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
 // an item's first part searches B's rows.
 #define HEAVY_MIN_PRODUCTS 65536
 #ifndef HEAVY_PART_COLS
-#define HEAVY_PART_COLS 4096         // 32 KiB of LDS per wave
+#define HEAVY_PART_COLS 2048         // 16 KiB of LDS per wave, seven waves per CU
 #endif
 #define HEAVY_ITEM_PRODUCTS 32768    // products per work item, about
 #define HEAVY_MAX_WAVES 4096
