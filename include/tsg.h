@@ -182,7 +182,7 @@ typedef struct {
                               LDS accumulator, smaller ones the compact
                               sparse one (default 64) */
     int s1_window_words;   /* step-1 LDS bitmap window, 32-bit words per
-                              wave: 256, 2048, or 0 = by B's width
+                              wave: 256, 1024, 2048, or 0 = by B's width
                               (default 0) */
     int force_tile_parallel; /* 1: steps 2 and 3 use the tile-parallel
                               kernels for every tile-row instead of the

@@ -2445,14 +2445,17 @@ static TileOperands make_operands(const tsg_dtiles *A, const tsg_dtiles *B) {
 }
 
 // Step-1 bitmap window in 32-bit words per wave: 256 (8192 tile columns,
-// 8 KiB of LDS per block) or 2048 (65536 tile columns, 32 KiB).  By default
-// the wide window serves B with 8193..65536 tile columns, where a row of a
-// product without band structure spans more than one narrow window; wider B
-// (banded operands such as stencils) keeps the narrow one.  The count pass's
-// bitmaps are kept for the fill pass up to 1 GiB.
+// 8 KiB of LDS per block), 1024 (32768, 16 KiB) or 2048 (65536, 32 KiB).
+// By default B with 8193..32768 tile columns gets the 1024-word window and
+// B with 32769..65536 the 2048-word one, so a row of a product without band
+// structure needs one window, with no more LDS (and, for the kept-row
+// bitmap ORs, registers) than that takes; wider B (banded operands such as
+// stencils) keeps the narrow one.  The count pass's bitmaps are kept for
+// the fill pass up to 1 GiB.
 int tsg_s1_words(const tsg_dtiles *B, const tsg_tune &t) {
-    if (t.s1_window_words == 256 || t.s1_window_words == 2048) return t.s1_window_words;
-    return B->tilen > 8192 && B->tilen <= 65536 ? 2048 : 256;
+    if (t.s1_window_words == 256 || t.s1_window_words == 1024 || t.s1_window_words == 2048) return t.s1_window_words;
+    if (B->tilen > 8192 && B->tilen <= 32768) return 1024;
+    return B->tilen > 32768 && B->tilen <= 65536 ? 2048 : 256;
 }
 static size_t keep_bytes(long long tilem, int words) {
     const size_t b = (size_t)tilem * (size_t)words * 4;
@@ -2494,7 +2497,16 @@ static int launch_step1(hipStream_t s, const tsg_dtiles *A, const tsg_dtiles *B,
     if (C->tilem <= 0) return TSG_OK;
     const dim3 g(tsg_div_up(C->tilem, S1_WAVES)), blk(64 * S1_WAVES);
     const bool wide = words == 2048;
-    if (!fill && !wide)
+    if (words == 1024) {
+        if (!fill)
+            hipLaunchKernelGGL((k_step1<false, 1024>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem,
+                               B->tile_ptr, B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, (int *)nullptr,
+                               (int *)nullptr, bmstore, rowspan);
+        else
+            hipLaunchKernelGGL((k_step1<true, 1024>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem,
+                               B->tile_ptr, B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, C->tile_colidx,
+                               C->tile_rowidx, bmstore, (int *)nullptr);
+    } else if (!fill && !wide)
         hipLaunchKernelGGL((k_step1<false, 256>), g, blk, 0, s, A->tile_ptr, A->tile_colidx, A->tilem, B->tile_ptr,
                            B->tile_colidx, bdesc, bbits, bany, C->tile_ptr, (int *)nullptr, (int *)nullptr, bmstore, rowspan);
     else if (!fill)
@@ -2767,8 +2779,9 @@ static int spgemm_run(tsg_ctx *ctx, const tsg_dtiles *A, const tsg_dtiles *B, co
     C->tilen = B->tilen;
     // ---- step 1
     TSG_TRY(tsg_alloc_t(ctx, &C->tile_ptr, (size_t)C->tilem + 1));
-    if (tune.s1_window_words != 0 && tune.s1_window_words != 256 && tune.s1_window_words != 2048) {
-        tsg_set_error("spgemm: s1_window_words %d is not 0, 256 or 2048", tune.s1_window_words);
+    if (tune.s1_window_words != 0 && tune.s1_window_words != 256 && tune.s1_window_words != 1024 &&
+        tune.s1_window_words != 2048) {
+        tsg_set_error("spgemm: s1_window_words %d is not 0, 256, 1024 or 2048", tune.s1_window_words);
         return TSG_ERR_ARG;
     }
     P.s1w = tsg_s1_words(B, tune);

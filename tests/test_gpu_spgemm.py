@@ -210,7 +210,7 @@ def test_repeat_is_deterministic(ctx):
 
 def test_wide_step1_window_rmat(ctx):
     """R-MAT scale 14 A*A^T, whose rows' candidates span the whole column
-    range: the check runs both step-1 windows (256 and 2048 words), all
+    range: the check runs every step-1 window (256, 1024 and 2048 words), all
     arrays identical to the oracle."""
     a = ts.gen_rmat(14, 8, seed=21)
     spgemm_check(ctx, a, t(a), label="rmat14 AAT windows")

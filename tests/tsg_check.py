@@ -76,8 +76,8 @@ def spgemm_check(ctx, a: ts.Csr, b: ts.Csr | None = None, tune=None, label="", v
     for k in ("tile_ptr", "tile_colidx", "tile_rowidx", "tile_nnz", "csr_ptr", "mask", "csr_col", "val"):
         assert np.array_equal(C2[k], C[k]), f"tile-parallel path differs in {k}"
     ctx.free(tc2)
-    # both step-1 bitmap windows give the same tile list
-    for words in (256, 2048):
+    # every step-1 bitmap window gives the same tile list
+    for words in (256, 1024, 2048):
         tw = ts.default_tune() if tune is None else type(tune).from_buffer_copy(tune)
         tw.s1_window_words = words
         tcw, _ = ctx.spgemm(ta, tb, tw)
