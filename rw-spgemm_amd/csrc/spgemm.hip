@@ -1187,8 +1187,21 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
 // stored + 1 (0 = empty), value = the entry's position in the segment.
 #define G3E_EHASH 4096  // >= 2 * G3E_NNZ
+// The wide (unpacked) variant's table: 2560 slots (load factor at most 0.6)
+// keep its LDS under 32,000 bytes, five blocks per CU like the packed one.
+#define G3E_EHASH_W 2560
 #define G3E_PF 8         // C tiles per thread whose metadata is loaded per round
 __device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 2654435761u) >> 20; }  // 12 bits
+// slot of key in a table of HS slots (4096, or the wide variant's 2560 by
+// multiply-shift range reduction), and the next slot
+template <unsigned HS>
+__device__ __forceinline__ unsigned ent_slot(unsigned key) {
+    return HS == 4096u ? ent_hash(key) : (unsigned)(((unsigned long long)(key * 2654435761u) * HS) >> 32);
+}
+template <unsigned HS>
+__device__ __forceinline__ unsigned ent_next(unsigned h) {
+    return (HS & (HS - 1)) == 0 ? (h + 1) & (HS - 1) : (h + 1 == HS ? 0u : h + 1);
+}
 // PK: when every tile-row spans fewer than 8192 tile columns the key (+1) is
 // below 2^21 and one 32-bit slot holds key << 11 | position (positions are
 // below G3E_NNZ <= 2048); without the separate position array the block's
@@ -1197,15 +1210,16 @@ __device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 26544
 static_assert(G3E_NNZ <= 2048, "positions take 11 bits of a packed hash slot");
 
 template <bool PK>
-__global__ void __launch_bounds__(256, PK ? 5 : 1)
+__global__ void __launch_bounds__(256, 5)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
              const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
              double *__restrict__ Cval, const unsigned char *__restrict__ heavy) {
     __shared__ double s_acc[G3E_NNZ];
-    __shared__ unsigned s_ekey[G3E_EHASH];
-    __shared__ unsigned short s_epos[PK ? 1 : G3E_EHASH];
+    constexpr unsigned HS = PK ? G3E_EHASH : G3E_EHASH_W;  // hash slots
+    __shared__ unsigned s_ekey[HS];
+    __shared__ unsigned short s_epos[PK ? 1 : HS];
     __shared__ unsigned char s_ecol[G3E_NNZ];  // in-tile column of each entry, final order
     const int tid = threadIdx.x;
     // the grid is an upper bound: the first *nseg blocks (spread evenly over
@@ -1219,7 +1233,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     const int nbase = Cnnz[c0], nseg_nnz = Cnnz[c0 + u] - nbase;
     const int col_lo = Ccol[c0] * 16, col_hi = Ccol[c0 + u - 1] * 16 + 16;
     for (int i = tid; i < nseg_nnz; i += 256) s_acc[i] = 0.0;
-    for (int i = tid; i < G3E_EHASH; i += 256) s_ekey[i] = 0u;
+    for (int i = tid; i < (int)HS; i += 256) s_ekey[i] = 0u;
     __syncthreads();
     // every C entry of the segment into the hash, and its in-tile column
     // into s_ecol (one C tile per thread; structurally empty tiles skipped).
@@ -1254,12 +1268,12 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     const int bb = 31 - __clz(cm);
                     cm &= ~(1u << bb);
                     const unsigned key = (((j16[v] + 15 - bb) << 4) | (unsigned)rr) + 1u;
-                    unsigned h = ent_hash(key);
+                    unsigned h = ent_slot<HS>(key);
                     if (PK) {
                         const unsigned ent = key << 11 | (unsigned)pos;
-                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = (h + 1) & (G3E_EHASH - 1);
+                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = ent_next<HS>(h);
                     } else {
-                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = (h + 1) & (G3E_EHASH - 1);  // nnz <= G3E_EHASH / 2
+                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = ent_next<HS>(h);  // nnz < HS
                         s_epos[h] = (unsigned short)pos;
                     }
                     s_ecol[pos] = (unsigned char)(15 - bb);
@@ -1282,18 +1296,18 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     // consistent C)
     auto acc_add = [&](int col, double a, double b) {
         const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
-        unsigned h = ent_hash(key);
+        unsigned h = ent_slot<HS>(key);
         int probe = 0;
         if (PK) {
             unsigned ent;
-            while (((ent = s_ekey[h]) >> 11) != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
-            if (probe < G3E_EHASH) {
+            while (((ent = s_ekey[h]) >> 11) != key && ++probe < (int)HS) h = ent_next<HS>(h);
+            if (probe < (int)HS) {
                 double *acc = &s_acc[ent & 2047u];
                 *acc = __builtin_fma(a, b, *acc);
             }
         } else {
-            while (s_ekey[h] != key && ++probe < G3E_EHASH) h = (h + 1) & (G3E_EHASH - 1);
-            if (probe < G3E_EHASH) {
+            while (s_ekey[h] != key && ++probe < (int)HS) h = ent_next<HS>(h);
+            if (probe < (int)HS) {
                 double *acc = &s_acc[s_epos[h]];
                 *acc = __builtin_fma(a, b, *acc);
             }
