@@ -1179,45 +1179,40 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 // ascending K, bits in ascending column) and, for each a_ik, its lanes
 // scatter the entries of B row k that fall in the segment's column range:
 // distinct C entries per step, steps in ascending k, so each C entry again
-// sums its products in the serial SPA order, without atomics.  1536
-// nonzeros keep the block's LDS (accumulator, hash, column bytes) under
-// 40,960 B, four blocks per CU in gfx950's 1,280-byte LDS granules.
+// sums its products in the serial SPA order, without atomics.  Segments of
+// 1536 nonzeros balance the re-walks of A rows per tile-row against the
+// block's LDS (accumulator, hash, column bytes), which sets the blocks per
+// CU in gfx950's 1,280-byte LDS granules.
 #define G3E_NNZ 1536
 
 // LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
 // stored + 1 (0 = empty), value = the entry's position in the segment.
-#define G3E_EHASH 4096  // >= 2 * G3E_NNZ
-// The wide (unpacked) variant's table: 2560 slots (load factor at most 0.6)
-// keep its LDS under 32,000 bytes, five blocks per CU like the packed one.
-#define G3E_EHASH_W 2560
-#define G3E_PF 8         // C tiles per thread whose metadata is loaded per round
-__device__ __forceinline__ unsigned ent_hash(unsigned key) { return (key * 2654435761u) >> 20; }  // 12 bits
-// slot of key in a table of HS slots (4096, or the wide variant's 2560 by
-// multiply-shift range reduction), and the next slot
-template <unsigned HS>
+// 2560 slots (load factor at most 0.6, linear probing) keep the packed
+// variant at 25.6 KB of LDS, six blocks per CU, and the unpacked one under
+// 32,000 bytes, five blocks per CU.
+#define G3E_EHASH 2560
+#define G3E_PF 4  // C tiles per thread whose metadata is loaded per round
+// home slot of a key (multiply-shift range reduction) and the next slot
 __device__ __forceinline__ unsigned ent_slot(unsigned key) {
-    return HS == 4096u ? ent_hash(key) : (unsigned)(((unsigned long long)(key * 2654435761u) * HS) >> 32);
+    return (unsigned)(((unsigned long long)(key * 2654435761u) * G3E_EHASH) >> 32);
 }
-template <unsigned HS>
-__device__ __forceinline__ unsigned ent_next(unsigned h) {
-    return (HS & (HS - 1)) == 0 ? (h + 1) & (HS - 1) : (h + 1 == HS ? 0u : h + 1);
-}
+__device__ __forceinline__ unsigned ent_next(unsigned h) { return h + 1 == G3E_EHASH ? 0u : h + 1; }
 // PK: when every tile-row spans fewer than 8192 tile columns the key (+1) is
 // below 2^21 and one 32-bit slot holds key << 11 | position (positions are
 // below G3E_NNZ <= 2048); without the separate position array the block's
-// LDS fits five blocks per CU.
+// LDS fits six blocks per CU.
 #define G3E_PK_SPAN 8192
 static_assert(G3E_NNZ <= 2048, "positions take 11 bits of a packed hash slot");
 
 template <bool PK>
-__global__ void __launch_bounds__(256, 5)
+__global__ void __launch_bounds__(256, PK ? 6 : 5)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
              const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
              double *__restrict__ Cval, const unsigned char *__restrict__ heavy) {
     __shared__ double s_acc[G3E_NNZ];
-    constexpr unsigned HS = PK ? G3E_EHASH : G3E_EHASH_W;  // hash slots
+    constexpr unsigned HS = G3E_EHASH;
     __shared__ unsigned s_ekey[HS];
     __shared__ unsigned short s_epos[PK ? 1 : HS];
     __shared__ unsigned char s_ecol[G3E_NNZ];  // in-tile column of each entry, final order
@@ -1268,12 +1263,12 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     const int bb = 31 - __clz(cm);
                     cm &= ~(1u << bb);
                     const unsigned key = (((j16[v] + 15 - bb) << 4) | (unsigned)rr) + 1u;
-                    unsigned h = ent_slot<HS>(key);
+                    unsigned h = ent_slot(key);
                     if (PK) {
                         const unsigned ent = key << 11 | (unsigned)pos;
-                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = ent_next<HS>(h);
+                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = ent_next(h);
                     } else {
-                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = ent_next<HS>(h);  // nnz < HS
+                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = ent_next(h);  // nnz < HS
                         s_epos[h] = (unsigned short)pos;
                     }
                     s_ecol[pos] = (unsigned char)(15 - bb);
@@ -1296,17 +1291,17 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     // consistent C)
     auto acc_add = [&](int col, double a, double b) {
         const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
-        unsigned h = ent_slot<HS>(key);
+        unsigned h = ent_slot(key);
         int probe = 0;
         if (PK) {
             unsigned ent;
-            while (((ent = s_ekey[h]) >> 11) != key && ++probe < (int)HS) h = ent_next<HS>(h);
+            while (((ent = s_ekey[h]) >> 11) != key && ++probe < (int)HS) h = ent_next(h);
             if (probe < (int)HS) {
                 double *acc = &s_acc[ent & 2047u];
                 *acc = __builtin_fma(a, b, *acc);
             }
         } else {
-            while (s_ekey[h] != key && ++probe < (int)HS) h = ent_next<HS>(h);
+            while (s_ekey[h] != key && ++probe < (int)HS) h = ent_next(h);
             if (probe < (int)HS) {
                 double *acc = &s_acc[s_epos[h]];
                 *acc = __builtin_fma(a, b, *acc);
