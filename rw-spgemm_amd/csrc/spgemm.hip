@@ -1186,17 +1186,24 @@ __device__ __forceinline__ void step3_tile_body(const TileOperands &op, const in
 #define G3E_NNZ 1536
 
 // LDS hash of the segment's C entries: key (column - col_lo) * 16 + row,
-// stored + 1 (0 = empty), value = the entry's position in the segment.
-// 2560 slots (load factor at most 0.6, linear probing) keep the packed
-// variant at 25.6 KB of LDS, six blocks per CU, and the unpacked one under
-// 32,000 bytes, five blocks per CU.
+// stored + 1 (0 = empty), value = the entry's position in the segment,
+// linear probing.  2560 slots (load factor at most 0.6): six blocks per CU
+// packed (25.6 KB of LDS; 24.2 KB with A's row index, which needs no
+// per-group decode scratch), five unpacked when A is decoded from its tiles
+// (30.7 KB).  Unpacked with A's row index, 2048 slots (at most 0.75) fit
+// six blocks (26.1 KB); the packed table stays at 2560, since at seven
+// blocks per CU its higher load factor cost more than the occupancy gave.
 #define G3E_EHASH 2560
+#define G3E_EHASH_RXW 2048
 #define G3E_PF 4  // C tiles per thread whose metadata is loaded per round
-// home slot of a key (multiply-shift range reduction) and the next slot
+// home slot of a key in a table of HS slots (multiply-shift range
+// reduction) and the next slot
+template <unsigned HS>
 __device__ __forceinline__ unsigned ent_slot(unsigned key) {
-    return (unsigned)(((unsigned long long)(key * 2654435761u) * G3E_EHASH) >> 32);
+    return (unsigned)(((unsigned long long)(key * 2654435761u) * HS) >> 32);
 }
-__device__ __forceinline__ unsigned ent_next(unsigned h) { return h + 1 == G3E_EHASH ? 0u : h + 1; }
+template <unsigned HS>
+__device__ __forceinline__ unsigned ent_next(unsigned h) { return h + 1 == HS ? 0u : h + 1; }
 // PK: when every tile-row spans fewer than 8192 tile columns the key (+1) is
 // below 2^21 and one 32-bit slot holds key << 11 | position (positions are
 // below G3E_NNZ <= 2048); without the separate position array the block's
@@ -1204,15 +1211,17 @@ __device__ __forceinline__ unsigned ent_next(unsigned h) { return h + 1 == G3E_E
 #define G3E_PK_SPAN 8192
 static_assert(G3E_NNZ <= 2048, "positions take 11 bits of a packed hash slot");
 
-template <bool PK>
-__global__ void __launch_bounds__(256, PK ? 6 : 5)
+// RX: A carries its element row index (rows' entries read directly); else
+// A row r's entries are decoded from its tiles.
+template <bool PK, bool RX>
+__global__ void __launch_bounds__(256, PK || RX ? 6 : 5)
 k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restrict__ Bcolx,
              const double *__restrict__ Bvalx, int m, const int4 *__restrict__ segs, const int *__restrict__ nseg,
              const int *__restrict__ Cptr, const int *__restrict__ Ccol, const unsigned short *__restrict__ Cmask,
              const unsigned char *__restrict__ Cptr16, const int *__restrict__ Cnnz, unsigned char *__restrict__ Ccol8,
              double *__restrict__ Cval, const unsigned char *__restrict__ heavy) {
     __shared__ double s_acc[G3E_NNZ];
-    constexpr unsigned HS = G3E_EHASH;
+    constexpr unsigned HS = RX && !PK ? G3E_EHASH_RXW : G3E_EHASH;
     __shared__ unsigned s_ekey[HS];
     __shared__ unsigned short s_epos[PK ? 1 : HS];
     __shared__ unsigned char s_ecol[G3E_NNZ];  // in-tile column of each entry, final order
@@ -1263,12 +1272,12 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                     const int bb = 31 - __clz(cm);
                     cm &= ~(1u << bb);
                     const unsigned key = (((j16[v] + 15 - bb) << 4) | (unsigned)rr) + 1u;
-                    unsigned h = ent_slot(key);
+                    unsigned h = ent_slot<HS>(key);
                     if (PK) {
                         const unsigned ent = key << 11 | (unsigned)pos;
-                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = ent_next(h);
+                        while (atomicCAS(&s_ekey[h], 0u, ent) != 0u) h = ent_next<HS>(h);
                     } else {
-                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = ent_next(h);  // nnz < HS
+                        while (atomicCAS(&s_ekey[h], 0u, key) != 0u) h = ent_next<HS>(h);  // nnz < HS
                         s_epos[h] = (unsigned short)pos;
                     }
                     s_ecol[pos] = (unsigned char)(15 - bb);
@@ -1284,31 +1293,31 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
     // this kernel and overwrites their values
     const bool skip = heavy && row < m && heavy[row];
     const int gb = tid & 48;  // first lane of this group in its wave
-    __shared__ int s_gex[16][16];             // per group: entries before each lane's tile
-    __shared__ unsigned short s_gam[16][16];  // per group: each lane's tile row mask
+    __shared__ int s_gex[RX ? 1 : 16][16];             // per group: entries before each lane's tile
+    __shared__ unsigned short s_gam[RX ? 1 : 16][16];  // per group: each lane's tile row mask
     // adds a * b into C entry (row 16I+r, col), located through the hash
     // (the entry is in C's structure; the probe bound is not reached for a
     // consistent C)
     auto acc_add = [&](int col, double a, double b) {
         const unsigned key = ((unsigned)(col - col_lo) << 4 | (unsigned)r) + 1u;
-        unsigned h = ent_slot(key);
+        unsigned h = ent_slot<HS>(key);
         int probe = 0;
         if (PK) {
             unsigned ent;
-            while (((ent = s_ekey[h]) >> 11) != key && ++probe < (int)HS) h = ent_next(h);
+            while (((ent = s_ekey[h]) >> 11) != key && ++probe < (int)HS) h = ent_next<HS>(h);
             if (probe < (int)HS) {
                 double *acc = &s_acc[ent & 2047u];
                 *acc = __builtin_fma(a, b, *acc);
             }
         } else {
-            while (s_ekey[h] != key && ++probe < (int)HS) h = ent_next(h);
+            while (s_ekey[h] != key && ++probe < (int)HS) h = ent_next<HS>(h);
             if (probe < (int)HS) {
                 double *acc = &s_acc[s_epos[h]];
                 *acc = __builtin_fma(a, b, *acc);
             }
         }
     };
-    if (row < m && nseg_nnz > 0 && op.Arowp && !skip) {
+    if (RX && row < m && nseg_nnz > 0 && !skip) {
         // A's row index: row 16I+r's entries are contiguous, ascending k.  The
         // 16 lanes take 16 of them (sources) and find their B ranges in
         // parallel; then the first 16 in-range B entries of all 16 sources are
@@ -1357,7 +1366,7 @@ k_step3_elem(TileOperands op, const int *__restrict__ Browp, const int *__restri
                 tsg_wave_sync();  // the next k of this row reads what lanes wrote
             }
         }
-    } else if (row < m && nseg_nnz > 0 && !skip) {
+    } else if (!RX && row < m && nseg_nnz > 0 && !skip) {
         const int a0 = op.Aptr[I], a1 = op.Aptr[I + 1];
         for (int t0 = a0; t0 < a1; t0 += 16) {
             // 16 A tiles per round, one per lane: their row-r entries are
@@ -2695,7 +2704,10 @@ static int launch_step3(hipStream_t s, const TileOperands &op, const tsg_dtiles 
     }
     if (!P.all && P.s3_unfit && P.s3_elem) {
         TSG_TRY(launch_segments<1>(s, C, w, 0, 0x7fffffff, G3E_NNZ, w.seg3, w.rowcls, w.seg3_cap));
-        hipLaunchKernelGGL(P.s3pk ? k_step3_elem<true> : k_step3_elem<false>, dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
+        const bool rx = op.Arowp != nullptr;
+        hipLaunchKernelGGL(P.s3pk ? (rx ? k_step3_elem<true, true> : k_step3_elem<true, false>)
+                                  : (rx ? k_step3_elem<false, true> : k_step3_elem<false, false>),
+                           dim3((unsigned)w.seg3_cap), dim3(256), 0, s, op, (const int *)B->rowx_ptr,
                            (const int *)B->rowx_col, (const double *)B->rowx_val, C->m, (const int4 *)w.seg3,
                            (const int *)(w.segcnt + C->tilem), C->tile_ptr, C->tile_colidx, C->mask, C->csr_ptr,
                            C->tile_nnz, C->csr_col, C->val, (const unsigned char *)w.hflag);
