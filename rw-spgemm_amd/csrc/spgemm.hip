@@ -225,7 +225,7 @@ k_step1(const int *__restrict__ Aptr, const int *__restrict__ Acol, int tilem,
     const bool kept = bmstore != nullptr && jmax >= jmin && jmax - jb < 32 * BMW;
     unsigned *keep = kept ? bmstore + (long long)I * BMW : nullptr;
     constexpr int V = BMW / 64;            // window words per lane: lane + 64 v
-    constexpr int U = BMW == 256 ? 4 : 1;  // kept B tile-rows whose words are loaded together
+    constexpr int U = BMW == 256 ? 2 : 1;  // kept B tile-rows whose words are loaded together
     for (int w0 = jb; w0 <= jmax; w0 += 32 * BMW) {
         const int wi = (w0 - jb) >> WSH;
         if (wi < 64 && !((wmask >> wi) & 1ull)) continue;  // no candidate in this window
